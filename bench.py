@@ -1,0 +1,144 @@
+#!/usr/bin/env python
+"""Benchmark of the GSIS kinetic loop on one MI355X.
+
+One "step" is one outer iteration of Hybrid_Time_advance_LU_SGS over the
+whole mesh and velocity grid (kinetic BCs, Shakhov RHS, LU-SGS sweep, moments,
+DETFP, Convert_NS).  Workload: the plain GSIS lid-driven cavity of the
+baseline (128 x 128 uniform cells, 64 x 64 non-uniform velocity grid dense
+near zero on [-6,6]^2, Kn = 0.1, lid 0.15, diffuse walls at T = 1, FP64),
+started from the gas at rest; synthetic data (no files).
+
+Prints one JSON line: throughput in cell-velocity updates/s (all inputs
+resident in HBM), a roofline entry for the dominant kernel timed with HIP
+events on the library's stream, and a CPU baseline (the CPU oracle on a
+bounded sample of the same workload, on this host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+PKG_DIR = ROOT / "hybrid2d-2--sis---shakov---coette--9-6_amd"
+HBM_PEAK_GBS = 8000.0
+
+
+def _load(name, path, pkg_dir=None):
+    if name in sys.modules:
+        return sys.modules[name]
+    kw = {"submodule_search_locations": [str(pkg_dir)]} if pkg_dir else {}
+    spec = importlib.util.spec_from_file_location(name, path, **kw)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def make_case(pkg, n, nv, kn, lid, dt):
+    return pkg.cases.cavity_case(n, nv, kn=kn, lid=lid, dt=dt, name=f"cavity{n}_v{nv}_kn{kn}")
+
+
+def cpu_baseline(case, steps):
+    """The oracle (CPU restatement, OpenMP over velocities) on `steps` steps of
+    the same case; returns cell-velocity updates/s."""
+    orc = _load("gsis_oracle_py", ROOT / "oracle" / "oracle.py")
+    o = orc.OracleSolver(case)
+    o.initialize()
+    t0 = time.perf_counter()
+    o.step(steps)
+    t1 = time.perf_counter()
+    o.close()
+    return case.ncell * case.nv * steps / (t1 - t0), t1 - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=128, help="cells per side")
+    ap.add_argument("--nv", type=int, default=64, help="velocities per direction")
+    ap.add_argument("--kn", type=float, default=0.1)
+    ap.add_argument("--lid", type=float, default=0.15)
+    ap.add_argument("--dt", type=float, default=1.0)
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps of the CPU-baseline sample (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the velocity-space sharding across GPUs is described in DESIGN.md, not built.", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = _load("gsis_amd", PKG_DIR / "__init__.py", PKG_DIR)
+    solver = _load("gsis_amd.solver", PKG_DIR / "solver.py")
+    case = make_case(pkg, args.n, args.nv, args.kn, args.lid, args.dt)
+    prec = pkg.capi.PREC_FP32 if args.precision == "fp32" else pkg.capi.PREC_FP64
+    s = solver.GsisSolver(case, precision=prec)
+    s.initialize()
+    print(f"# case {case.name}: {case.ncell} cells x {case.nv} velocities, precision {args.precision}", flush=True)
+    s.step(args.warmup)
+    s.counters()  # synchronise the library's stream
+    s.timing(1)
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.counters()
+    t1 = time.perf_counter()
+    ms, nt = s.timing(0)
+    res = s.residual()
+    s.close()
+    el = t1 - t0
+    units = case.ncell * case.nv
+    value = units * args.steps / el
+    esz = 4 if args.precision == "fp32" else 8
+    # algorithmic HBM bytes per launch of each kinetic kernel (per cell-velocity):
+    #   rhs: read F (G,H) + write RHS (G,H); sweep: read RHS + read F + write F;
+    #   moments: read F.  The whole step's algorithmic figure is 4*esz
+    #   (one read and one write of G and H).
+    per_unit = {"rhs": 4 * esz, "sweep": 6 * esz, "moments": 2 * esz}
+    names = ["rhs", "sweep", "moments"]
+    avg_ms = [m / max(nt, 1) for m in ms]
+    k = max(range(3), key=lambda q: avg_ms[q])
+    ring_units = (case.nx + 1) * (case.ny + 1) * case.nv
+    bytes_launch = per_unit[names[k]] * (ring_units if names[k] == "moments" else units)
+    achieved = bytes_launch / (avg_ms[k] * 1e-3) / 1e9
+    step_alg = 4 * esz * units / (el / args.steps) / 1e9
+    out = {
+        "metric": "cell·velocity updates/s + time-to-converge vs plain GSIS at 1e-4 rel L2",
+        "value": value,
+        "unit": "cell·velocity updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-sweep/f64-moments",
+        "data": "synthetic",
+        "config": {"workload": f"plain GSIS lid-driven cavity {args.n}x{args.n} cells, {args.nv}x{args.nv} sinh "
+                               f"velocity grid on [-6,6]^2, Kn={args.kn}, lid {args.lid}, dt={args.dt}, "
+                               f"{args.precision}, from rest",
+                   "cells": case.ncell, "velocities": case.nv},
+        "roofline": {"bound": "hbm", "kernel": names[k], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms[k],
+                     "kernel_ms": dict(zip(names, avg_ms)),
+                     "step_algorithmic_GBs": step_alg, "step_frac": step_alg / HBM_PEAK_GBS},
+        "residual": list(map(float, res)),
+    }
+    if args.cpu_steps > 0:
+        ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        v, secs = cpu_baseline(case, args.cpu_steps)
+        out["cpu_baseline"] = {"value": v, "unit": "cell·velocity updates/s", "cores": ncores, "kind": "port",
+                               "sample": f"{args.cpu_steps} steps of the same case on the C oracle "
+                                         f"({secs:.1f} s, OpenMP {ncores} threads)"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

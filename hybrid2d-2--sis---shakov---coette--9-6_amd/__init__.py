@@ -1,0 +1,13 @@
+"""MI355X-native GSIS kinetic loop (discrete-velocity Shakhov solve + synthetic
+NS correction) behind the C-ABI of include/gsis_capi.h.
+
+Submodules:
+  capi    -- ctypes mirror of the C-ABI and the libgsis.so loader
+  cases   -- host-side producers: mesh geometry, velocity grids, constants,
+             initial state, readers of the reference's input files
+  solver  -- GsisSolver: the driver loop over the C-ABI
+  build   -- hipcc build of libgsis.so for gfx950
+"""
+from . import capi, cases  # noqa: F401
+
+__all__ = ["capi", "cases"]

@@ -1,0 +1,502 @@
+// gsis_capi.hip -- host side of the C-ABI declared in include/gsis_capi.h.
+//
+// Owns the device state of one GSIS case and orders the kernels of one outer
+// iteration exactly as Hybrid_Time_advance_LU_SGS does (src/Hybrid_Sub.f90:4-160):
+//   S0<-S, kinetic BCs in input order, kinetic RHS + LU-SGS sweep, moments +
+//   DETFP + DETFPwall, Convert_NS (U, HoT), [synthetic NS], counters.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gsis_capi.h"
+#include "gsis_kernels.h"
+#include "gsis_synthetic.h"
+
+using gsis::BcDev;
+using gsis::Dev;
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+}  // namespace
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(GSIS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));              \
+  } while (0)
+
+struct gsis_handle {
+  gsis_params p{};
+  std::vector<gsis_bc> bc;
+  std::vector<BcDev> bcd;
+  Dev d{};
+  hipStream_t s = nullptr;
+  int prec = 0;
+  size_t tsize = 8;
+  void *FG = nullptr, *FH = nullptr, *slot = nullptr, *RG = nullptr, *RH = nullptr;
+  std::vector<void *> allocs;
+  int *wall_list = nullptr;
+  int nwall = 0;
+  double *swall = nullptr;
+  int swall_len = 0;
+  double *resbuf = nullptr;
+  int kstep = 0;
+  double tt = 0.0;
+  int ns_iter = 0;
+  gsis::SyntheticState syn{};
+  // optional kernel timing (gsis_timing): events around the kinetic kernels
+  bool timing = false;
+  hipEvent_t ev[6] = {};
+  double t_ms[3] = {0, 0, 0};
+  int t_n = 0;
+  bool t_pending = false;
+};
+
+namespace {
+
+template <typename P>
+int dalloc(gsis_handle *h, P **ptr, size_t bytes) {
+  void *q = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(GSIS_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemset(q, 0, bytes);
+  if (e != hipSuccess) return fail(GSIS_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  h->allocs.push_back(q);
+  *ptr = reinterpret_cast<P *>(q);
+  return 0;
+}
+
+int bc_nk(const gsis_bc &b) { return (b.face == 1 || b.face == 3) ? (b.jend - b.jst + 1) : (b.iend - b.ist + 1); }
+
+int validate(const gsis_params *p) {
+  if (!p) return fail(GSIS_EARG, "params is NULL");
+  if (p->nx < 5 || p->ny < 5) return fail(GSIS_EARG, "nx and ny must be >= 5 nodes");
+  if (p->nvi < 1 || p->nvj < 1 || p->nvi > gsis::kMaxV1 || p->nvj > gsis::kMaxV1)
+    return fail(GSIS_EARG, "nvi/nvj out of range (1..512)");
+  if (!p->vi || !p->vj || !p->wi || !p->wj || !p->x || !p->y || !p->vol || !p->si || !p->sj || !p->ni1 ||
+      !p->ni2 || !p->nj1 || !p->nj2)
+    return fail(GSIS_EARG, "a required array pointer is NULL");
+  if (p->nbc < 0 || (p->nbc > 0 && !p->bc)) return fail(GSIS_EARG, "bad BC list");
+  if (p->scheme != GSIS_SCHEME_UP1 && p->scheme != GSIS_SCHEME_NND)
+    return fail(GSIS_EUNSUPPORTED, "only the UP1 and NND kinetic schemes are implemented");
+  if (p->precision != GSIS_PREC_FP64 && p->precision != GSIS_PREC_FP32)
+    return fail(GSIS_EARG, "precision must be GSIS_PREC_FP64 or GSIS_PREC_FP32");
+  if (p->mode != GSIS_MODE_M0 && p->mode != GSIS_MODE_M1) return fail(GSIS_EARG, "unknown mode");
+  if (p->dt <= 0.0) return fail(GSIS_EARG, "dt must be positive");
+  for (int b = 0; b < p->nbc; b++) {
+    const gsis_bc &c = p->bc[b];
+    bool ok = false;
+    if (c.face == 1) ok = c.ist == 1 && c.iend == 1;
+    if (c.face == 3) ok = c.ist == p->nx && c.iend == p->nx;
+    if (c.face == 2) ok = c.jst == 1 && c.jend == 1;
+    if (c.face == 4) ok = c.jst == p->ny && c.jend == p->ny;
+    if (!ok) return fail(GSIS_EARG, "BC " + std::to_string(b) + ": face does not match its index range");
+    if (c.ist < 1 || c.iend > p->nx || c.jst < 1 || c.jend > p->ny || c.ist > c.iend || c.jst > c.jend)
+      return fail(GSIS_EARG, "BC " + std::to_string(b) + ": index range outside the block");
+    if (c.code != GSIS_BC_WALL && c.code != GSIS_BC_FARFIELD && c.code != GSIS_BC_PERIODIC)
+      return fail(GSIS_EUNSUPPORTED, "BC " + std::to_string(b) + ": only wall, farfield and periodic are implemented");
+  }
+  return 0;
+}
+
+template <typename T>
+void phy_boundary(gsis_handle *h) {
+  T *FG = (T *)h->FG, *FH = (T *)h->FH, *slot = (T *)h->slot;
+  for (const BcDev &b : h->bcd) {
+    if (b.code == GSIS_BC_WALL) gsis::launch_wall<T>(h->d, b, FG, FH, slot, h->s);
+    else if (b.code == GSIS_BC_FARFIELD) gsis::launch_farfield<T>(h->d, b, FG, FH, h->s);
+    else if (b.code == GSIS_BC_PERIODIC) gsis::launch_periodic<T>(h->d, b, FG, FH, h->s);
+  }
+}
+
+template <typename T>
+void moments(gsis_handle *h, bool timed = false) {
+  if (timed && h->timing) (void)hipEventRecord(h->ev[4], h->s);
+  gsis::launch_moments<T>(h->d, (T *)h->FG, (T *)h->FH, h->s);
+  if (timed && h->timing) { (void)hipEventRecord(h->ev[5], h->s); h->t_pending = true; }
+  gsis::launch_det_wall(h->d, h->wall_list, h->nwall, h->swall, h->s);
+}
+
+void harvest_timing(gsis_handle *h) {
+  if (!h->t_pending) return;
+  float ms = 0.f;
+  for (int k = 0; k < 3; k++)
+    if (hipEventElapsedTime(&ms, h->ev[2 * k], h->ev[2 * k + 1]) == hipSuccess) h->t_ms[k] += ms;
+  h->t_n += 1;
+  h->t_pending = false;
+}
+
+template <typename T>
+void kinetic(gsis_handle *h) {
+  if (h->timing) {
+    if (h->t_pending) { (void)hipEventSynchronize(h->ev[5]); harvest_timing(h); }
+    (void)hipEventRecord(h->ev[0], h->s);
+  }
+  gsis::launch_rhs<T>(h->d, (const T *)h->FG, (const T *)h->FH, (const T *)h->slot, (T *)h->RG, (T *)h->RH, h->s);
+  if (h->timing) { (void)hipEventRecord(h->ev[1], h->s); (void)hipEventRecord(h->ev[2], h->s); }
+  gsis::launch_sweep<T>(h->d, (T *)h->FG, (T *)h->FH, (const T *)h->RG, (const T *)h->RH, h->s);
+  if (h->timing) (void)hipEventRecord(h->ev[3], h->s);
+}
+
+template <typename T>
+int one_step(gsis_handle *h) {
+  gsis::launch_save_s0(h->d, h->s);
+  phy_boundary<T>(h);
+  kinetic<T>(h);
+  moments<T>(h, true);
+  gsis::launch_convert(h->d, h->s);
+  if (h->p.if_viscous) gsis::launch_hot(h->d, h->s);
+  if (h->p.mode == GSIS_MODE_M1) {
+    int rc = gsis::synthetic_step(h->d, h->syn, h->s, &h->ns_iter);
+    if (rc) return fail(rc, gsis::synthetic_error());
+  }
+  HIPCHK(hipGetLastError());
+  h->tt += h->p.dt;
+  h->kstep += 1;
+  return 0;
+}
+
+int check_err_flag(gsis_handle *h) {
+  int e = 0;
+  HIPCHK(hipMemcpyAsync(&e, h->d.err, sizeof(int), hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  if (e & 1) return fail(GSIS_ETEMP, "T < 0.1 in an interior cell (comput_duvtpckw stop)");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gsis_last_error(void) { return g_err.c_str(); }
+
+int gsis_create(const gsis_params *p, gsis_handle **out) {
+  if (!out) return fail(GSIS_EARG, "out is NULL");
+  *out = nullptr;
+  int rc = validate(p);
+  if (rc) return rc;
+  gsis_handle *h = new gsis_handle();
+  h->p = *p;
+  h->bc.assign(p->bc, p->bc + p->nbc);
+  h->prec = p->precision;
+  h->tsize = (p->precision == GSIS_PREC_FP32) ? sizeof(float) : sizeof(double);
+  const int nx = p->nx, ny = p->ny;
+  Dev &d = h->d;
+  d.nx = nx; d.ny = ny; d.nvi = p->nvi; d.nvj = p->nvj; d.nv = p->nvi * p->nvj;
+  d.NI = nx + 3; d.NJ = ny + 3; d.ncell = d.NI * d.NJ; d.GI = nx + 2;
+  d.L = (nx > ny ? nx : ny) + 3;
+  d.cfc = p->cfc; d.x1s = p->x1s; d.cpr = p->cpr; d.twall = p->twall; d.dt = p->dt;
+  d.ma = p->ma; d.gamma = p->gamma; d.re = p->re; d.pr = p->pr; d.cv = p->cv; d.cp = p->cp; d.aoa = p->aoa;
+  d.scheme = p->scheme;
+  auto cleanup = [&](int code) {
+    gsis_destroy(h);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(GSIS_EHIP, "hipStreamCreate failed"));
+  // velocity grid
+  double *vi, *vj, *wi, *wj;
+  if ((rc = dalloc(h, &vi, 8 * p->nvi)) || (rc = dalloc(h, &wi, 8 * p->nvi)) || (rc = dalloc(h, &vj, 8 * p->nvj)) ||
+      (rc = dalloc(h, &wj, 8 * p->nvj)))
+    return cleanup(rc);
+  if (hipMemcpy(vi, p->vi, 8 * p->nvi, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(wi, p->wi, 8 * p->nvi, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(vj, p->vj, 8 * p->nvj, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(wj, p->wj, 8 * p->nvj, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(GSIS_EHIP, "velocity grid upload failed"));
+  d.vi = vi; d.vj = vj; d.wi = wi; d.wj = wj;
+  // geometry (nx,ny) Fortran -> zero-padded (nx+2)x(ny+2)
+  const size_t ng = (size_t)(nx + 2) * (ny + 2);
+  const double *src[7] = {p->vol, p->si, p->sj, p->ni1, p->ni2, p->nj1, p->nj2};
+  const double *dst[7];
+  std::vector<double> tmp(ng);
+  for (int a = 0; a < 7; a++) {
+    std::fill(tmp.begin(), tmp.end(), 0.0);
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) tmp[(size_t)j * (nx + 2) + i] = src[a][(size_t)(i - 1) + (size_t)nx * (j - 1)];
+    double *g;
+    if ((rc = dalloc(h, &g, 8 * ng))) return cleanup(rc);
+    if (hipMemcpy(g, tmp.data(), 8 * ng, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(GSIS_EHIP, "geometry upload failed"));
+    dst[a] = g;
+  }
+  d.vol = dst[0]; d.si = dst[1]; d.sj = dst[2]; d.ni1 = dst[3]; d.ni2 = dst[4]; d.nj1 = dst[5]; d.nj2 = dst[6];
+  // node coordinates: Fortran x(-1:nx+2,-1:ny+2) has exactly the device layout
+  const size_t nn = (size_t)(nx + 4) * (ny + 4);
+  double *xn, *yn;
+  if ((rc = dalloc(h, &xn, 8 * nn)) || (rc = dalloc(h, &yn, 8 * nn))) return cleanup(rc);
+  if (hipMemcpy(xn, p->x, 8 * nn, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(yn, p->y, 8 * nn, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(GSIS_EHIP, "coordinate upload failed"));
+  d.xn = xn; d.yn = yn;
+  // cell centres (src/sub_init.f90:814-819)
+  std::vector<double> cx(d.ncell), cy(d.ncell);
+  for (int j = -1; j <= ny + 1; j++)
+    for (int i = -1; i <= nx + 1; i++) {
+      auto X = [&](const double *a, int ii, int jj) { return a[(size_t)(ii + 1) + (size_t)(nx + 4) * (jj + 1)]; };
+      const int c = (j + 1) * d.NI + (i + 1);
+      cx[c] = (X(p->x, i, j) + X(p->x, i + 1, j) + X(p->x, i, j + 1) + X(p->x, i + 1, j + 1)) * 0.25;
+      cy[c] = (X(p->y, i, j) + X(p->y, i + 1, j) + X(p->y, i, j + 1) + X(p->y, i + 1, j + 1)) * 0.25;
+    }
+  double *x1, *y1;
+  if ((rc = dalloc(h, &x1, 8 * d.ncell)) || (rc = dalloc(h, &y1, 8 * d.ncell))) return cleanup(rc);
+  if (hipMemcpy(x1, cx.data(), 8 * d.ncell, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(y1, cy.data(), 8 * d.ncell, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(GSIS_EHIP, "cell centre upload failed"));
+  d.x1 = x1; d.y1 = y1;
+  // macroscopic state
+  const size_t N = d.ncell;
+  if ((rc = dalloc(h, &d.S, 8 * 11 * N)) || (rc = dalloc(h, &d.S0, 8 * 10 * N)) || (rc = dalloc(h, &d.CF, 8 * N)) ||
+      (rc = dalloc(h, &d.U, 8 * 4 * N)) || (rc = dalloc(h, &d.U0, 8 * 4 * N)) || (rc = dalloc(h, &d.U2, 8 * 4 * N)) ||
+      (rc = dalloc(h, &d.taoi, 8 * 4 * N)) || (rc = dalloc(h, &d.taoj, 8 * 4 * N)) ||
+      (rc = dalloc(h, &d.qi, 8 * 2 * N)) || (rc = dalloc(h, &d.qj, 8 * 2 * N)) ||
+      (rc = dalloc(h, &d.prim, 8 * 5 * N)) || (rc = dalloc(h, &d.res4, 8 * 4 * N)) || (rc = dalloc(h, &d.err, 64)) ||
+      (rc = dalloc(h, &h->resbuf, 8 * 8)))
+    return cleanup(rc);
+  {  // Creat_Mesh sets U, U0, U2 to (1,0,0,1) (src/sub_init.f90:159-162)
+    std::vector<double> u(4 * N, 0.0);
+    for (size_t c = 0; c < N; c++) { u[c] = 1.0; u[3 * N + c] = 1.0; }
+    if (hipMemcpy(d.U, u.data(), 8 * 4 * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.U0, u.data(), 8 * 4 * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.U2, u.data(), 8 * 4 * N, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(GSIS_EHIP, "U upload failed"));
+  }
+  // face override flags and the DETFPwall node list
+  std::vector<uint8_t> oi1(ny + 2, 0), oi3(ny + 2, 0), oj2(nx + 2, 0), oj4(nx + 2, 0);
+  std::vector<int> wl;
+  int soff = 0;
+  for (int b = 0; b < p->nbc; b++) {
+    const gsis_bc &c = p->bc[b];
+    BcDev bd{c.face, c.ist, c.iend, c.jst, c.jend, c.code, c.uw, c.vw, -1};
+    if (c.code == GSIS_BC_WALL || c.code == GSIS_BC_FARFIELD) {
+      bd.swall_off = soff;
+      if (c.face == 1) for (int j = c.jst; j <= c.jend - 1; j++) oi1[j] = 1;
+      if (c.face == 3) for (int j = c.jst; j <= c.jend - 1; j++) oi3[j] = 1;
+      if (c.face == 2) for (int i = c.ist; i <= c.iend - 1; i++) oj2[i] = 1;
+      if (c.face == 4) for (int i = c.ist; i <= c.iend - 1; i++) oj4[i] = 1;
+      const int nk = bc_nk(c);
+      for (int kk = 0; kk < nk - 1; kk++) {  // DETFPwall skips the last node
+        int g1, g2, a1, a2, b1, b2;
+        if (c.face == 1) { int j = c.jst + kk; g1 = c.ist - 1; g2 = j; a1 = c.ist; a2 = j; b1 = c.ist + 1; b2 = j; }
+        else if (c.face == 2) { int i = c.ist + kk; g1 = i; g2 = c.jst - 1; a1 = i; a2 = c.jst; b1 = i; b2 = c.jst + 1; }
+        else if (c.face == 3) { int j = c.jst + kk; g1 = c.iend; g2 = j; a1 = c.iend - 1; a2 = j; b1 = c.iend - 2; b2 = j; }
+        else { int i = c.ist + kk; g1 = i; g2 = c.jend; a1 = i; a2 = c.jend - 1; b1 = i; b2 = c.jend - 2; }
+        wl.push_back(gsis::cidx(d, g1, g2));
+        wl.push_back(gsis::cidx(d, a1, a2));
+        wl.push_back(gsis::cidx(d, b1, b2));
+        wl.push_back(soff + 10 * kk);
+      }
+      soff += 10 * nk;
+    }
+    h->bcd.push_back(bd);
+  }
+  h->swall_len = soff;
+  h->nwall = (int)wl.size() / 4;
+  uint8_t *o1, *o3, *o2, *o4;
+  if ((rc = dalloc(h, &o1, ny + 2)) || (rc = dalloc(h, &o3, ny + 2)) || (rc = dalloc(h, &o2, nx + 2)) ||
+      (rc = dalloc(h, &o4, nx + 2)) || (rc = dalloc(h, &h->wall_list, 4 * wl.size())) ||
+      (rc = dalloc(h, &h->swall, 8 * (size_t)soff)))
+    return cleanup(rc);
+  if (hipMemcpy(o1, oi1.data(), ny + 2, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(o3, oi3.data(), ny + 2, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(o2, oj2.data(), nx + 2, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(o4, oj4.data(), nx + 2, hipMemcpyHostToDevice) != hipSuccess ||
+      (!wl.empty() && hipMemcpy(h->wall_list, wl.data(), 4 * wl.size(), hipMemcpyHostToDevice) != hipSuccess))
+    return cleanup(fail(GSIS_EHIP, "BC table upload failed"));
+  d.ovrI1 = o1; d.ovrI3 = o3; d.ovrJ2 = o2; d.ovrJ4 = o4;
+  // distributions
+  const size_t nf = (size_t)d.ncell * d.nv * h->tsize;
+  const size_t ns = (size_t)4 * d.L * 2 * d.nv * h->tsize;
+  if ((rc = dalloc(h, &h->FG, nf)) || (rc = dalloc(h, &h->FH, nf)) || (rc = dalloc(h, &h->slot, ns)) ||
+      (rc = dalloc(h, &h->RG, nf)) || (rc = dalloc(h, &h->RH, nf)))
+    return cleanup(rc);
+  if (p->mode == GSIS_MODE_M1) {
+    rc = gsis::synthetic_init(h->d, *p, h->syn);
+    if (rc) return cleanup(fail(rc, gsis::synthetic_error()));
+  }
+  *out = h;
+  return 0;
+}
+
+int gsis_timing(gsis_handle *h, int32_t enable, double *ms, int32_t *nsteps) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  if (enable && !h->ev[0])
+    for (int k = 0; k < 6; k++) HIPCHK(hipEventCreate(&h->ev[k]));
+  HIPCHK(hipStreamSynchronize(h->s));
+  harvest_timing(h);
+  if (ms) for (int k = 0; k < 3; k++) ms[k] = h->t_ms[k];
+  if (nsteps) *nsteps = h->t_n;
+  if (enable >= 0) {
+    h->timing = enable != 0;
+    for (int k = 0; k < 3; k++) h->t_ms[k] = 0.0;
+    h->t_n = 0;
+  }
+  return 0;
+}
+
+int gsis_destroy(gsis_handle *h) {
+  if (!h) return 0;
+  if (h->s) (void)hipStreamSynchronize(h->s);
+  for (int k = 0; k < 6; k++)
+    if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+  gsis::synthetic_free(h->syn);
+  for (void *q : h->allocs) (void)hipFree(q);
+  if (h->s) (void)hipStreamDestroy(h->s);
+  delete h;
+  return 0;
+}
+
+int gsis_upload_frd(gsis_handle *h, const double *frd) {
+  if (!h || !frd) return fail(GSIS_EARG, "NULL argument");
+  const Dev &d = h->d;
+  const size_t n = (size_t)2 * (d.nx + 7) * (d.ny + 7) * d.nv;
+  double *tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, n * 8));
+  hipError_t e = hipMemcpyAsync(tmp, frd, n * 8, hipMemcpyHostToDevice, h->s);
+  if (e == hipSuccess) {
+    if (h->prec == GSIS_PREC_FP32) gsis::launch_frd_in<float>(d, tmp, (float *)h->FG, (float *)h->FH, (float *)h->slot, h->s);
+    else gsis::launch_frd_in<double>(d, tmp, (double *)h->FG, (double *)h->FH, (double *)h->slot, h->s);
+    e = hipStreamSynchronize(h->s);
+  }
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(GSIS_EHIP, std::string("upload_frd: ") + hipGetErrorString(e));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int gsis_download_frd(gsis_handle *h, double *frd) {
+  if (!h || !frd) return fail(GSIS_EARG, "NULL argument");
+  const Dev &d = h->d;
+  const size_t n = (size_t)2 * (d.nx + 7) * (d.ny + 7) * d.nv;
+  double *tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, n * 8));
+  hipError_t e = hipMemsetAsync(tmp, 0, n * 8, h->s);
+  if (e == hipSuccess) {
+    if (h->prec == GSIS_PREC_FP32) gsis::launch_frd_out<float>(d, tmp, (const float *)h->FG, (const float *)h->FH, (const float *)h->slot, h->s);
+    else gsis::launch_frd_out<double>(d, tmp, (const double *)h->FG, (const double *)h->FH, (const double *)h->slot, h->s);
+    e = hipMemcpyAsync(frd, tmp, n * 8, hipMemcpyDeviceToHost, h->s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(GSIS_EHIP, std::string("download_frd: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int gsis_upload_macro(gsis_handle *h, const double *s, const double *cf, const double *u, const double *u0) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  const Dev &d = h->d;
+  const size_t N = d.ncell;
+  HIPCHK(hipStreamSynchronize(h->s));
+  if (s) {
+    std::vector<double> t(11 * N, 0.0);
+    for (int j = -1; j <= d.ny + 1; j++)
+      for (int i = -1; i <= d.nx + 1; i++) {
+        const int c = gsis::cidx(d, i, j);
+        for (int k = 0; k < 11; k++) t[k * N + c] = s[k + 11 * ((size_t)(i + 3) + (size_t)(d.nx + 7) * (j + 3))];
+      }
+    HIPCHK(hipMemcpy(d.S, t.data(), 8 * 11 * N, hipMemcpyHostToDevice));
+  }
+  if (cf) HIPCHK(hipMemcpy(d.CF, cf, 8 * N, hipMemcpyHostToDevice));
+  double *targets[2] = {d.U, d.U0};
+  const double *srcs[2] = {u, u0};
+  for (int a = 0; a < 2; a++) {
+    if (!srcs[a]) continue;
+    std::vector<double> t(4 * N);
+    for (size_t c = 0; c < N; c++)
+      for (int m = 0; m < 4; m++) t[m * N + c] = srcs[a][m + 4 * c];
+    HIPCHK(hipMemcpy(targets[a], t.data(), 8 * 4 * N, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int gsis_download_macro(gsis_handle *h, double *s, double *s0, double *cf, double *u, double *taonsi_g,
+                        double *taonsj_g, double *qi_g, double *qj_g, double *swall) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  const Dev &d = h->d;
+  const size_t N = d.ncell;
+  HIPCHK(hipStreamSynchronize(h->s));
+  int rc = check_err_flag(h);
+  auto get_s = [&](const double *dev, int ncomp, double *out) -> int {
+    std::vector<double> t(ncomp * N);
+    HIPCHK(hipMemcpy(t.data(), dev, 8 * ncomp * N, hipMemcpyDeviceToHost));
+    const size_t tot = (size_t)11 * (d.nx + 7) * (d.ny + 7);
+    std::memset(out, 0, 8 * tot);
+    for (int j = -1; j <= d.ny + 1; j++)
+      for (int i = -1; i <= d.nx + 1; i++) {
+        const int c = gsis::cidx(d, i, j);
+        for (int k = 0; k < ncomp; k++) out[k + 11 * ((size_t)(i + 3) + (size_t)(d.nx + 7) * (j + 3))] = t[k * N + c];
+      }
+    return 0;
+  };
+  auto get_v = [&](const double *dev, int ncomp, double *out) -> int {
+    std::vector<double> t(ncomp * N);
+    HIPCHK(hipMemcpy(t.data(), dev, 8 * ncomp * N, hipMemcpyDeviceToHost));
+    for (size_t c = 0; c < N; c++)
+      for (int m = 0; m < ncomp; m++) out[m + ncomp * c] = t[m * N + c];
+    return 0;
+  };
+  int r2 = 0;
+  if (s && (r2 = get_s(d.S, 11, s))) return r2;
+  if (s0 && (r2 = get_s(d.S0, 10, s0))) return r2;
+  if (cf) HIPCHK(hipMemcpy(cf, d.CF, 8 * N, hipMemcpyDeviceToHost));
+  if (u && (r2 = get_v(d.U, 4, u))) return r2;
+  if (taonsi_g && (r2 = get_v(d.taoi, 4, taonsi_g))) return r2;
+  if (taonsj_g && (r2 = get_v(d.taoj, 4, taonsj_g))) return r2;
+  if (qi_g && (r2 = get_v(d.qi, 2, qi_g))) return r2;
+  if (qj_g && (r2 = get_v(d.qj, 2, qj_g))) return r2;
+  if (swall && h->swall_len > 0) HIPCHK(hipMemcpy(swall, h->swall, 8 * (size_t)h->swall_len, hipMemcpyDeviceToHost));
+  return rc;
+}
+
+int gsis_refresh_moments(gsis_handle *h) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  if (h->prec == GSIS_PREC_FP32) { phy_boundary<float>(h); moments<float>(h); }
+  else { phy_boundary<double>(h); moments<double>(h); }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->s));
+  return 0;
+}
+
+int gsis_step(gsis_handle *h, int32_t nsteps) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  if (nsteps < 0) return fail(GSIS_EARG, "nsteps < 0");
+  for (int k = 0; k < nsteps; k++) {
+    int rc = (h->prec == GSIS_PREC_FP32) ? one_step<float>(h) : one_step<double>(h);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int gsis_residual(gsis_handle *h, double res_rms[4]) {
+  if (!h || !res_rms) return fail(GSIS_EARG, "NULL argument");
+  gsis::launch_residual(h->d, h->resbuf, h->s);
+  double r[8];
+  HIPCHK(hipMemcpyAsync(r, h->resbuf, sizeof(r), hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  for (int k = 0; k < 4; k++) res_rms[k] = r[k];
+  int rc = check_err_flag(h);
+  if (rc) return rc;
+  if (std::isnan(res_rms[0])) return fail(GSIS_ENAN, "NaN detected in the residual");
+  return 0;
+}
+
+int gsis_counters(gsis_handle *h, int32_t *kstep, double *tt, int32_t *ns_iter) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  HIPCHK(hipStreamSynchronize(h->s));
+  if (kstep) *kstep = h->kstep;
+  if (tt) *tt = h->tt;
+  if (ns_iter) *ns_iter = h->ns_iter;
+  return 0;
+}
+
+}  // extern "C"

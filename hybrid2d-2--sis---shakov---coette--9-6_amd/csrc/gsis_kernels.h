@@ -1,0 +1,23 @@
+// gsis_kernels.h -- launch wrappers of the gfx950 kernels (gsis_kernels.hip).
+#pragma once
+#include "gsis_device.h"
+
+namespace gsis {
+
+constexpr int kMaxV1 = 512;  // max discrete velocities per direction (LDS tables)
+
+template <typename T> void launch_wall(const Dev &d, const BcDev &b, T *FG, T *FH, T *slot, hipStream_t s);
+template <typename T> void launch_farfield(const Dev &d, const BcDev &b, T *FG, T *FH, hipStream_t s);
+template <typename T> void launch_periodic(const Dev &d, const BcDev &b, T *FG, T *FH, hipStream_t s);
+template <typename T> void launch_rhs(const Dev &d, const T *FG, const T *FH, const T *slot, T *RG, T *RH, hipStream_t s);
+template <typename T> void launch_sweep(const Dev &d, T *FG, T *FH, const T *RG, const T *RH, hipStream_t s);
+template <typename T> void launch_moments(const Dev &d, T *FG, T *FH, hipStream_t s);
+template <typename T> void launch_frd_in(const Dev &d, const double *frd, T *FG, T *FH, T *slot, hipStream_t s);
+template <typename T> void launch_frd_out(const Dev &d, double *frd, const T *FG, const T *FH, const T *slot, hipStream_t s);
+void launch_save_s0(const Dev &d, hipStream_t s);
+void launch_det_wall(const Dev &d, const int *list, int n, double *swall, hipStream_t s);
+void launch_convert(const Dev &d, hipStream_t s);
+void launch_hot(const Dev &d, hipStream_t s);
+void launch_residual(const Dev &d, double *out, hipStream_t s);
+
+}  // namespace gsis

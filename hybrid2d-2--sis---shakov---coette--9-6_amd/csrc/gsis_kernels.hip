@@ -1,0 +1,922 @@
+// gsis_kernels.hip -- hand-written gfx950 kernels of one GSIS outer iteration.
+//
+// Every kernel names the reference routine whose arithmetic it carries; the
+// layout is described in gsis_device.h.  Storage/compute type T is double for
+// the FP64 path and float for the FP32-sweep path (moments, wall sums and the
+// macroscopic fields stay FP64 in both).
+#include "gsis_kernels.h"
+
+namespace gsis {
+
+// ------------------------------------------------------------------------
+// small helpers
+// ------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T fminmd(T x, T y) {  // FMINMD, src/Functions.f90:10-26
+  if ((x > T(0) && y > T(0)) || (x < T(0) && y < T(0))) return (fabs(x) <= fabs(y)) ? x : y;
+  return T(0);
+}
+template <typename T>
+__device__ __forceinline__ T posp(T a) { return (a + fabs(a)) * T(0.5); }
+template <typename T>
+__device__ __forceinline__ T negp(T a) { return (a - fabs(a)) * T(0.5); }
+
+// Block-wide sum of N doubles (wave64 shuffles, then one LDS pass).  All
+// threads receive the totals.  blockDim.x must be a multiple of 64, <= 1024.
+template <int N>
+__device__ void block_sum(double (&v)[N]) {
+  __shared__ double sh[N][16];
+#pragma unroll
+  for (int k = 0; k < N; k++)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; k++) sh[k][wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double s = 0.0;
+    for (int w = 0; w < nw; w++) s += sh[threadIdx.x][w];
+    sh[threadIdx.x][0] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) v[k] = sh[k][0];
+  __syncthreads();
+}
+
+__device__ int block_min_int(int x) {
+  __shared__ int shm[16];
+  for (int off = 32; off > 0; off >>= 1) x = min(x, __shfl_down(x, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) shm[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = shm[0];
+    for (int w = 1; w < nw; w++) m = min(m, shm[w]);
+    shm[0] = m;
+  }
+  __syncthreads();
+  int r = shm[0];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------------------
+// S0 <- S at the start of the step (src/Hybrid_Sub.f90:29-38)
+// ------------------------------------------------------------------------
+__global__ void k_save_s0(Dev d) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int ncl = (d.nx - 1) * (d.ny - 1);
+  if (t >= ncl) return;
+  int i = t % (d.nx - 1) + 1, j = t / (d.nx - 1) + 1;
+  int c = cidx(d, i, j);
+  for (int k = 0; k < 10; k++) d.S0[k * d.ncell + c] = d.S[k * d.ncell + c];
+}
+
+// ------------------------------------------------------------------------
+// Diffuse wall: ghost-slot extrapolation (GKUA_boundary_wall,
+// src/GKUA_boundary.f90:69-97), wall density (NWGaussLegendre,
+// src/GKUA_wall_functions.f90:106-199) and incoming Maxwellian
+// (GKUA_wall_boundary, src/GKUA_wall_functions.f90:27-63).
+// One block per face node (the last node of the face is skipped, as in the
+// reference's icheck/jcheck test).
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_wall(Dev d, BcDev b, T *FG, T *FH, T *slot) {
+  const int kk = blockIdx.x;
+  int i, j, px, py, px1, py1, pos;
+  double swnx, swny, st;
+  if (b.face == 1) { i = b.ist; j = b.jst + kk; px = 1; py = j; px1 = 2; py1 = j; pos = j + 1; }
+  else if (b.face == 2) { i = b.ist + kk; j = b.jst; px = i; py = 1; px1 = i; py1 = 2; pos = i + 1; }
+  else if (b.face == 3) { i = b.iend; j = b.jst + kk; px = d.nx - 1; py = j; px1 = d.nx - 2; py1 = j; pos = j + 1; }
+  else { i = b.ist + kk; j = b.jend; px = i; py = d.ny - 1; px1 = i; py1 = d.ny - 2; pos = i + 1; }
+  if (b.face == 1 || b.face == 3) { swnx = d.ni1[gidx(d, i, j)]; swny = d.ni2[gidx(d, i, j)]; }
+  else { swnx = d.nj1[gidx(d, i, j)]; swny = d.nj2[gidx(d, i, j)]; }
+  st = (b.face <= 2) ? 1.0 : -1.0;
+  const size_t c0 = (size_t)cidx(d, px, py) * d.nv, c1 = (size_t)cidx(d, px1, py1) * d.nv;
+  const double tw = d.twall;
+  double acc[2] = {0.0, 0.0};
+  int first_in = 0x7fffffff;
+  for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
+    const int iv = v % d.nvi, jv = v / d.nvi;
+    T outg, outh;
+    {
+      T f1 = FG[c0 + v], f2 = FG[c1 + v];
+      T t = (-f2 + T(3) * f1) * T(0.5);
+      if (t <= T(0)) t = f1;
+      if (t >= T(1.5) * f1) t = T(1.5) * f1;
+      outg = t;
+      f1 = FH[c0 + v]; f2 = FH[c1 + v];
+      t = (-f2 + T(3) * f1) * T(0.5);
+      if (t <= T(0)) t = f1;
+      if (t >= T(1.5) * f1) t = T(1.5) * f1;
+      outh = t;
+    }
+    slot[slot_idx(d, b.face, pos, 0, v)] = outg;
+    slot[slot_idx(d, b.face, pos, 1, v)] = outh;
+    const double w = d.wi[iv] * d.wj[jv];
+    const double cx = d.vi[iv] - b.uw, cy = d.vj[jv] - b.vw;
+    const double vnw = (cx * swnx + cy * swny) * st;
+    const double vnn = (vnw - fabs(vnw)) / 2.;
+    acc[0] += w * (vnn * (double)outg);
+    acc[1] += w * ((vnw + fabs(vnw)) / 2. * exp(-(cx * cx + cy * cy) / tw) / (kPi * tw));
+    if (vnw >= 0.0) first_in = min(first_in, v);
+  }
+  block_sum<2>(acc);
+  first_in = block_min_int(first_in);
+  const double rho = -acc[0] / acc[1];
+  for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
+    const int iv = v % d.nvi, jv = v / d.nvi;
+    const double cx = d.vi[iv] - b.uw, cy = d.vj[jv] - b.vw;
+    const double vnw = (cx * swnx + cy * swny) * st;
+    if (vnw >= 0.0) {
+      // the reference clamps a negative wall density only after its first use
+      // (src/GKUA_wall_functions.f90:52-54)
+      const double r = (v == first_in || rho >= 0.0) ? rho : 0.0;
+      const double g = r * exp(-(cx * cx + cy * cy) / tw) / (kPi * tw);
+      slot[slot_idx(d, b.face, pos, 0, v)] = T(g);
+      slot[slot_idx(d, b.face, pos, 1, v)] = T(tw * g / 2.);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Kinetic far field: Maxwellian ghosts from the NS state
+// (GKUA_boundary_farfield, src/GKUA_boundary.f90:103-215).
+// One block per (face node, layer k).
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_farfield(Dev d, BcDev b, T *FG, T *FH) {
+  const int nn = (b.face == 1 || b.face == 3) ? (b.jend - b.jst + 1) : (b.iend - b.ist + 1);
+  const int m = blockIdx.x % nn, k = blockIdx.x / nn + 1;
+  int i, j;
+  if (b.face == 1 || b.face == 3) { i = b.ist; j = b.jst + m; } else { i = b.ist + m; j = b.jst; }
+  int i1, j1, i2, j2;
+  if (b.face == 1 || b.face == 2) { i1 = i; j1 = j; }
+  else if (b.face == 3) { i1 = i - 1; j1 = j; }
+  else { i1 = i; j1 = j - 1; }
+  if (b.face == 1) { i2 = i - k; j2 = j; }
+  else if (b.face == 2) { i2 = i; j2 = j - k; }
+  else if (b.face == 3) { i2 = i + k - 1; j2 = j; }
+  else { i2 = i; j2 = j + k - 1; }
+  double dx, dy, s, n1, n2;
+  if (b.face == 1 || b.face == 3) {
+    dx = d.xn[nidx(d, i, j + 1)] - d.xn[nidx(d, i, j)];
+    dy = d.yn[nidx(d, i, j + 1)] - d.yn[nidx(d, i, j)];
+    s = sqrt(dx * dx + dy * dy);
+    if (b.face == 1) { n1 = -dy / s; n2 = dx / s; } else { n1 = dy / s; n2 = -dx / s; }
+  } else {
+    dx = d.xn[nidx(d, i + 1, j)] - d.xn[nidx(d, i, j)];
+    dy = d.yn[nidx(d, i + 1, j)] - d.yn[nidx(d, i, j)];
+    s = sqrt(dx * dx + dy * dy);
+    if (b.face == 2) { n1 = dy / s; n2 = -dx / s; } else { n1 = -dy / s; n2 = dx / s; }
+  }
+  const double g = d.gamma, ma = d.ma;
+  const double d_inf = 1., u_inf = cos(d.aoa), v_inf = sin(d.aoa), p_inf = 1. / (g * ma * ma);
+  const double ma_n = (u_inf * n1 + v_inf * n2) * ma;
+  const int ca = cidx(d, i1, j1), cb = cidx(d, i2, j2);
+  double u4[4];
+  if (ma_n > 0.) {
+    for (int q = 0; q < 4; q++) u4[q] = d.U[q * d.ncell + ca];
+  } else {
+    u4[0] = d_inf; u4[1] = d_inf * u_inf; u4[2] = d_inf * v_inf;
+    u4[3] = p_inf / (g - 1.) + 0.5 * d_inf * (u_inf * u_inf + v_inf * v_inf);
+  }
+  const double rho = u4[0], tu = u4[1] / rho, tv = u4[2] / rho;
+  const double sq = sqrt(2.0 / g);
+  const double n0 = u4[0];
+  const double uu0 = (u4[1] / rho) * ma / sq, vv0 = (u4[2] / rho) * ma / sq;
+  const double tt = (u4[3] - 0.5 * rho * (tu * tu + tv * tv)) / (d.cv * rho);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; q++) d.U[q * d.ncell + cb] = u4[q];
+    d.S[0 * d.ncell + cb] = n0;
+    d.S[1 * d.ncell + cb] = uu0;
+    d.S[2 * d.ncell + cb] = vv0;
+    d.S[3 * d.ncell + cb] = tt;
+    d.S[4 * d.ncell + cb] = tt;
+  }
+  const size_t cf = (size_t)cb * d.nv;
+  for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
+    const int iv = v % d.nvi, jv = v / d.nvi;
+    const double viu = d.vi[iv] - uu0, vjv = d.vj[jv] - vv0;
+    const double vij2 = (viu * viu + vjv * vjv) / tt;
+    const double gd = n0 / (kPi * tt) * exp(-vij2);
+    FG[cf + v] = T(gd);
+    FH[cf + v] = T(gd * tt / 2.);
+  }
+}
+
+// ------------------------------------------------------------------------
+// Periodic copy ("inlet", src/GKUA_boundary.f90:219-273): one block per
+// (face node, layer).
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void k_periodic(Dev d, BcDev b, T *FG, T *FH) {
+  const int nn = (b.face == 1 || b.face == 3) ? (b.jend - b.jst + 1) : (b.iend - b.ist + 1);
+  const int m = blockIdx.x % nn, k = blockIdx.x / nn + 1;
+  int i, j;
+  if (b.face == 1 || b.face == 3) { i = b.ist; j = b.jst + m; } else { i = b.ist + m; j = b.jst; }
+  int i1, j1, i2, j2;
+  if (b.face == 1) { i1 = d.nx - k; j1 = j; i2 = i - k; j2 = j; }
+  else if (b.face == 2) { i1 = i; j1 = d.ny - k; i2 = i; j2 = j - k; }
+  else if (b.face == 3) { i1 = k; j1 = j; i2 = i + k - 1; j2 = j; }
+  else { i1 = i; j1 = k; i2 = i; j2 = j + k - 1; }
+  const int ca = cidx(d, i1, j1), cb = cidx(d, i2, j2);
+  if (threadIdx.x < 11) d.S[threadIdx.x * d.ncell + cb] = d.S[threadIdx.x * d.ncell + ca];
+  const size_t sa = (size_t)ca * d.nv, sb = (size_t)cb * d.nv;
+  for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
+    FG[sb + v] = FG[sa + v];
+    FH[sb + v] = FH[sa + v];
+  }
+}
+
+// ------------------------------------------------------------------------
+// Kinetic RHS for every interior cell and velocity: reduced Shakhov
+// equilibrium (Get_Feq, src/GKUA_Residual.f90:420-469), NND upwind fluxes with
+// the wall/farfield face overrides and the BGK-Shakhov source (RHSFDVsub,
+// src/GKUA_Residual.f90:117-418).  Also reduces the four conserved moments
+// of the RHS per cell (GaussLegendre_rs, src/GKUA_DETFP_RES.f90:81-120).
+// One block per interior cell; threads stride over velocities.
+// ------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T flux_nnd(T a, T fm2, T fm1, T f0, T fp1, bool nnd) {
+  const T asp = posp(a), asn = negp(a);
+  if (!nnd) return asp * fm1 + asn * f0;
+  const T d0 = f0 - fm1;
+  const T ulg = fm1 + T(0.5) * fminmd(d0, fm1 - fm2);
+  const T urg = f0 - T(0.5) * fminmd(d0, fp1 - f0);
+  return asp * ulg + asn * urg;
+}
+// face next to a wall/farfield face on the low side: unlimited left state
+// (src/GKUA_Residual.f90:214-220)
+template <typename T>
+__device__ __forceinline__ T flux_lowadj(T a, T f1, T f2, T f3, bool nnd) {
+  const T asp = posp(a), asn = negp(a);
+  if (!nnd) return asp * f1 + asn * f2;
+  const T ulg = f1 + T(0.5) * (f2 - f1);
+  const T urg = f2 - T(0.5) * fminmd(f2 - f1, f3 - f2);
+  return asp * ulg + asn * urg;
+}
+// face next to a wall/farfield face on the high side: unlimited right state
+// (src/GKUA_Residual.f90:251-257); f1 = F(n-1), f2 = F(n-2), f3 = F(n-3)
+template <typename T>
+__device__ __forceinline__ T flux_highadj(T a, T f1, T f2, T f3, bool nnd) {
+  const T asp = posp(a), asn = negp(a);
+  if (!nnd) return asp * f2 + asn * f1;
+  const T ulg = f2 + T(0.5) * fminmd(f1 - f2, f2 - f3);
+  const T urg = f1 - T(0.5) * (f1 - f2);
+  return asp * ulg + asn * urg;
+}
+
+// flux through i-face fi of row j (between cells fi-1 and fi); stencil
+// F(fi-2), F(fi-1), F(fi), F(fi+1)
+template <typename T>
+__device__ __forceinline__ T iface_flux(const Dev &d, int fi, int j, T a, T fm2, T fm1, T f0, T fp1,
+                                        const T *slot, int ird, int v, bool nnd) {
+  if (d.ovrI1[j]) {
+    if (fi == 1) return a * slot[slot_idx(d, 1, j + 1, ird, v)];
+    if (fi == 2) return flux_lowadj(a, fm1, f0, fp1, nnd);
+  }
+  if (d.ovrI3[j]) {
+    if (fi == d.nx) return a * slot[slot_idx(d, 3, j + 1, ird, v)];
+    if (fi == d.nx - 1) return flux_highadj(a, f0, fm1, fm2, nnd);
+  }
+  return flux_nnd(a, fm2, fm1, f0, fp1, nnd);
+}
+template <typename T>
+__device__ __forceinline__ T jface_flux(const Dev &d, int i, int fj, T a, T fm2, T fm1, T f0, T fp1,
+                                        const T *slot, int ird, int v, bool nnd) {
+  if (d.ovrJ2[i]) {
+    if (fj == 1) return a * slot[slot_idx(d, 2, i + 1, ird, v)];
+    if (fj == 2) return flux_lowadj(a, fm1, f0, fp1, nnd);
+  }
+  if (d.ovrJ4[i]) {
+    if (fj == d.ny) return a * slot[slot_idx(d, 4, i + 1, ird, v)];
+    if (fj == d.ny - 1) return flux_highadj(a, f0, fm1, fm2, nnd);
+  }
+  return flux_nnd(a, fm2, fm1, f0, fp1, nnd);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_rhs(Dev d, const T *FG, const T *FH, const T *slot, T *RG, T *RH) {
+  __shared__ T ex[kMaxV1], ey[kMaxV1];
+  const int t = blockIdx.x;
+  const int i = t % (d.nx - 1) + 1, j = t / (d.nx - 1) + 1;
+  const int c = cidx(d, i, j);
+  const double n = d.S[0 * d.ncell + c], uu = d.S[1 * d.ncell + c], vv = d.S[2 * d.ncell + c];
+  const double tt = d.S[3 * d.ncell + c], qx = d.S[7 * d.ncell + c], qy = d.S[8 * d.ncell + c];
+  // exp(-|c|^2/T) = exp(-cx^2/T) exp(-cy^2/T): nvi+nvj exponentials per cell
+  for (int q = threadIdx.x; q < d.nvi; q += blockDim.x) {
+    const double cx = d.vi[q] - uu;
+    ex[q] = T(exp(-(cx * cx) / tt));
+  }
+  for (int q = threadIdx.x; q < d.nvj; q += blockDim.x) {
+    const double cy = d.vj[q] - vv;
+    ey[q] = T(exp(-(cy * cy) / tt));
+  }
+  __syncthreads();
+  const T crt = T(n / (kPi * tt));
+  const T cpt = T(d.cpr / (n * tt * tt));
+  const T Tt = T(tt), Uu = T(uu), Vv = T(vv), Qx = T(qx), Qy = T(qy);
+  const T vol = T(d.vol[gidx(d, i, j)]), cf = T(d.CF[c]);
+  const double sil = d.si[gidx(d, i, j)], sir = d.si[gidx(d, i + 1, j)];
+  const double nil1 = d.ni1[gidx(d, i, j)], nil2 = d.ni2[gidx(d, i, j)];
+  const double nir1 = d.ni1[gidx(d, i + 1, j)], nir2 = d.ni2[gidx(d, i + 1, j)];
+  const double sjb = d.sj[gidx(d, i, j)], sjt = d.sj[gidx(d, i, j + 1)];
+  const double njb1 = d.nj1[gidx(d, i, j)], njb2 = d.nj2[gidx(d, i, j)];
+  const double njt1 = d.nj1[gidx(d, i, j + 1)], njt2 = d.nj2[gidx(d, i, j + 1)];
+  const bool nnd = d.scheme != 0;
+  const size_t nv = d.nv;
+  const size_t o0 = (size_t)c * nv;
+  const size_t oim2 = (size_t)cidx(d, i - 2, j) * nv, oim1 = (size_t)cidx(d, i - 1, j) * nv;
+  const size_t oip1 = (size_t)cidx(d, i + 1, j) * nv, oip2 = (size_t)cidx(d, i + 2, j) * nv;
+  const size_t ojm2 = (size_t)cidx(d, i, j - 2) * nv, ojm1 = (size_t)cidx(d, i, j - 1) * nv;
+  const size_t ojp1 = (size_t)cidx(d, i, j + 1) * nv, ojp2 = (size_t)cidx(d, i, j + 2) * nv;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
+    const int iv = v % d.nvi, jv = v / d.nvi;
+    const double vxd = d.vi[iv], vyd = d.vj[jv];
+    const T axl = T((vxd * nil1 + vyd * nil2) * sil), axr = T((vxd * nir1 + vyd * nir2) * sir);
+    const T ayb = T((vxd * njb1 + vyd * njb2) * sjb), ayt = T((vxd * njt1 + vyd * njt2) * sjt);
+    // Shakhov equilibrium of this cell (Get_Feq)
+    const T viu = T(vxd) - Uu, vjv = T(vyd) - Vv;
+    const T vij2 = (viu * viu + vjv * vjv) / Tt;
+    const T fvx = cpt * (viu * Qx + vjv * Qy);
+    const T gd = crt * (ex[iv] * ey[jv]);
+    const T feqg = gd * (T(1) + fvx * (T(2) * vij2 - T(4)));
+    const T feqh = (Tt * gd / T(2)) * (T(1) + fvx * (T(2) * vij2 - T(2)));
+    T rhs[2];
+#pragma unroll
+    for (int ird = 0; ird < 2; ird++) {
+      const T *F = ird == 0 ? FG : FH;
+      const T f0 = F[o0 + v];
+      const T fim2 = F[oim2 + v], fim1 = F[oim1 + v], fip1 = F[oip1 + v], fip2 = F[oip2 + v];
+      const T fjm2 = F[ojm2 + v], fjm1 = F[ojm1 + v], fjp1 = F[ojp1 + v], fjp2 = F[ojp2 + v];
+      const T flx_l = iface_flux(d, i, j, axl, fim2, fim1, f0, fip1, slot, ird, v, nnd);
+      const T flx_r = iface_flux(d, i + 1, j, axr, fim1, f0, fip1, fip2, slot, ird, v, nnd);
+      const T fly_b = jface_flux(d, i, j, ayb, fjm2, fjm1, f0, fjp1, slot, ird, v, nnd);
+      const T fly_t = jface_flux(d, i, j + 1, ayt, fjm1, f0, fjp1, fjp2, slot, ird, v, nnd);
+      const T net = ((flx_r - flx_l) + fly_t) - fly_b;
+      const T feq = ird == 0 ? feqg : feqh;
+      rhs[ird] = -net / vol + cf * (feq - f0);
+    }
+    RG[o0 + v] = rhs[0];
+    RH[o0 + v] = rhs[1];
+    const double w = d.wi[iv] * d.wj[jv];
+    const double g = (double)rhs[0], h = (double)rhs[1];
+    const double vx1 = vxd * g, vy1 = vyd * g;
+    acc[0] += w * g;
+    acc[1] += w * vx1;
+    acc[2] += w * vy1;
+    acc[3] += w * (h + vxd * vx1 + vyd * vy1);
+  }
+  block_sum<4>(acc);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; k++) d.res4[k * d.ncell + c] = acc[k];
+}
+
+// ------------------------------------------------------------------------
+// LU-SGS sweep of the kinetic equation (src/GKUA_Residual.f90:34-109):
+// forward recursion on anti-diagonal planes i+j=p, the non-recursive
+// backward pass (it reads forward values only, :82-86), update F+=dF and
+// the positivity clamp of interior cells.  One block per tile of VT
+// consecutive velocities walks all planes; the forward values of the last
+// three planes live in LDS.  F is updated in place (the RHS kernel already
+// consumed the old values).
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_sweep(Dev d, int VT, T *FG, T *FH, const T *RG, const T *RH) {
+  extern __shared__ unsigned char lds_raw[];
+  T *D = reinterpret_cast<T *>(lds_raw);  // [3][nx+1][VT][2]
+  const int lane = threadIdx.x % VT, slot_s = threadIdx.x / VT, NS = blockDim.x / VT;
+  const int v = blockIdx.x * VT + lane;
+  const bool act = v < d.nv;
+  const int vv = act ? v : 0;
+  const double vxd = d.vi[vv % d.nvi], vyd = d.vj[vv / d.nvi];
+  const int nx = d.nx, ny = d.ny;
+  const T dt = T(d.dt);
+  const int plast = (nx - 1) + (ny - 1);
+  auto didx = [&](int ring, int col, int ird) { return ((ring * (nx + 1) + col) * VT + lane) * 2 + ird; };
+  auto ax = [&](int i, int j) { const int g = gidx(d, i, j); return T((vxd * d.ni1[g] + vyd * d.ni2[g]) * d.si[g]); };
+  auto ay = [&](int i, int j) { const int g = gidx(d, i, j); return T((vxd * d.nj1[g] + vyd * d.nj2[g]) * d.sj[g]); };
+  for (int p = 2; p <= plast + 1; p++) {
+    if (p <= plast) {
+      const int ilo = max(1, p - (ny - 1)), ihi = min(nx - 1, p - 1);
+      const int ring = p % 3, rprev = (p - 1) % 3;
+      for (int k = slot_s; k <= ihi - ilo; k += NS) {
+        const int i = ilo + k, j = p - i;
+        const T axi = ax(i, j), axi1 = ax(i + 1, j), ayj = ay(i, j), ayj1 = ay(i, j + 1);
+        const T vol = T(d.vol[gidx(d, i, j)]);
+        const int c = cidx(d, i, j);
+        const T alfa = vol / dt + (posp(axi1) - negp(axi) + posp(ayj1) - negp(ayj)) + vol * T(d.CF[c]);
+        const T axp1 = posp(axi), ayp1 = posp(ayj);
+        const size_t o = (size_t)c * d.nv + vv;
+#pragma unroll
+        for (int ird = 0; ird < 2; ird++) {
+          const T dl = (i - 1 >= 1) ? D[didx(rprev, i - 1, ird)] : T(0);
+          const T db = (j - 1 >= 1) ? D[didx(rprev, i, ird)] : T(0);
+          const T r = act ? (ird == 0 ? RG[o] : RH[o]) : T(0);
+          D[didx(ring, i, ird)] = (vol * r + axp1 * dl + ayp1 * db) / alfa;
+        }
+      }
+    }
+    __syncthreads();
+    const int q = p - 1;
+    if (q >= 2) {
+      const int ilo = max(1, q - (ny - 1)), ihi = min(nx - 1, q - 1);
+      const int rq = q % 3, rn = p % 3;
+      for (int k = slot_s; k <= ihi - ilo; k += NS) {
+        const int i = ilo + k, j = q - i;
+        const T axi = ax(i, j), axi1 = ax(i + 1, j), ayj = ay(i, j), ayj1 = ay(i, j + 1);
+        const T vol = T(d.vol[gidx(d, i, j)]);
+        const int c = cidx(d, i, j);
+        const T alfa = vol / dt + (posp(axi1) - negp(axi) + posp(ayj1) - negp(ayj)) + vol * T(d.CF[c]);
+        const T axn1 = negp(axi1), ayn1 = negp(ayj1);
+        const size_t o = (size_t)c * d.nv + vv;
+#pragma unroll
+        for (int ird = 0; ird < 2; ird++) {
+          const T dr = (i + 1 <= nx - 1) ? D[didx(rn, i + 1, ird)] : T(0);
+          const T dtp = (j + 1 <= ny - 1) ? D[didx(rn, i, ird)] : T(0);
+          const T du = D[didx(rq, i, ird)] + (-axn1 * dr + -ayn1 * dtp) / alfa;
+          if (act) {
+            T *F = ird == 0 ? FG : FH;
+            T f = F[o] + du;
+            if ((double)f < kClamp) f = T(kClamp);
+            F[o] = f;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Positivity clamp of the ghost ring (src/GKUA_Residual.f90:101-109) and the
+// velocity moments with the DETFP conversion and collision frequency
+// (GaussLegendre src/GKUA_DETFP.f90:288-363, DETFP :52-105).  The reference
+// zeroes all of S_GKUA before the moments (:303), so cells outside
+// [0,nx]x[0,ny] get 0.  One block per padded cell.
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_moments(Dev d, T *FG, T *FH) {
+  const int c = blockIdx.x;
+  const int i = c % d.NI - 1, j = c / d.NI - 1;
+  const bool interior = (i >= 1 && i <= d.nx - 1 && j >= 1 && j <= d.ny - 1);
+  const bool ring = (i >= 0 && i <= d.nx && j >= 0 && j <= d.ny);
+  const size_t o = (size_t)c * d.nv;
+  double acc[10];
+#pragma unroll
+  for (int k = 0; k < 10; k++) acc[k] = 0.0;
+  for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
+    T g = FG[o + v], h = FH[o + v];
+    if (!interior) {
+      if ((double)g < kClamp) { g = T(kClamp); FG[o + v] = g; }
+      if ((double)h < kClamp) { h = T(kClamp); FH[o + v] = h; }
+    }
+    if (ring) {
+      const int iv = v % d.nvi, jv = v / d.nvi;
+      const double vx = d.vi[iv], vy = d.vj[jv], w = d.wi[iv] * d.wj[jv];
+      const double gd = (double)g, hd = (double)h;
+      const double vx1 = vx * gd, vy1 = vy * gd, vx2 = vx * vx1, vy2 = vy * vy1;
+      const double f4 = hd + vx2 + vy2;
+      acc[0] += w * gd;
+      acc[1] += w * vx1;
+      acc[2] += w * vy1;
+      acc[3] += w * f4;
+      acc[4] += w * vx2;
+      acc[5] += w * vy2;
+      acc[6] += w * (vx * vy * gd);
+      acc[7] += w * (vx * f4);
+      acc[8] += w * (vy * f4);
+      acc[9] += w * hd;
+    }
+  }
+  if (!ring) {
+    if (threadIdx.x < 11) d.S[threadIdx.x * d.ncell + c] = 0.0;
+    return;  // uniform per block: no barrier below is skipped by part of it
+  }
+  block_sum<10>(acc);
+  if (threadIdx.x == 0) {
+    double ron = acc[0];
+    if (ron <= 0.) ron = 1.0;  // R00
+    const double u = acc[1] / ron, v = acc[2] / ron;
+    const double us1 = u * u, vs1 = v * v, sum_uv = us1 + vs1;
+    double ttr = (acc[3] - ron * sum_uv) / (1.5 * ron);
+    if (ttr <= 0.) ttr = 1.0;  // T00
+    double *S = d.S;
+    const int N = d.ncell;
+    S[0 * N + c] = acc[0];
+    S[1 * N + c] = u;
+    S[2 * N + c] = v;
+    S[3 * N + c] = ttr;
+    S[4 * N + c] = (acc[4] - ron * us1) * 2.;
+    S[5 * N + c] = (acc[5] - ron * vs1) * 2.;
+    S[6 * N + c] = (acc[6] - ron * u * v) * 2.;
+    S[7 * N + c] = acc[7] - 2. * u * acc[4] - 2. * v * acc[6] + ron * u * (sum_uv - 1.5 * ttr);
+    S[8 * N + c] = acc[8] - 2. * v * acc[5] - 2. * u * acc[6] + ron * v * (sum_uv - 1.5 * ttr);
+    S[9 * N + c] = acc[9] * 2.;
+    S[10 * N + c] = 0.0;
+    if (d.x1s <= 9.99999993922529e-09) d.CF[c] = d.cfc * acc[0];  // REAL(4) 1.e-8
+    else d.CF[c] = d.cfc * acc[0] * pow(ttr, d.x1s);
+  }
+}
+
+// DETFPwall for every wall/farfield face node (src/GKUA_DETFP_WALL.f90:34-95).
+// list[4*k..]: ghost cell, first cell, second cell, Swall offset.
+__global__ void k_det_wall(Dev d, const int *list, int n, double *swall) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int cg = list[4 * t], ca = list[4 * t + 1], cb = list[4 * t + 2], so = list[4 * t + 3];
+  for (int k = 0; k < 10; k++) {
+    const double sw = (-d.S[k * d.ncell + cb] + 3.0 * d.S[k * d.ncell + ca]) * 0.5;
+    swall[so + k] = sw;
+    d.S[k * d.ncell + cg] = 2.0 * sw - d.S[k * d.ncell + ca];
+  }
+}
+
+// Convert_NS part 1 (src/Hybrid_Sub.f90:221-235): corner extrapolation of S
+// and the NS conservative variables on [0,nx]x[0,ny].
+__global__ void k_convert_u(Dev d) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n1 = d.nx + 1;
+  if (t >= n1 * (d.ny + 1)) return;
+  const int i = t % n1, j = t / n1;
+  const int nx = d.nx, ny = d.ny, N = d.ncell;
+  const int c = cidx(d, i, j);
+  double *S = d.S;
+  int ca = -1, cb = -1, cc = -1;
+  if (i == 0 && j == 0) { ca = cidx(d, 1, 0); cb = cidx(d, 0, 1); cc = cidx(d, 1, 1); }
+  else if (i == nx && j == 0) { ca = cidx(d, nx, 1); cb = cidx(d, nx - 1, 0); cc = cidx(d, nx - 1, 1); }
+  else if (i == 0 && j == ny) { ca = cidx(d, 1, ny); cb = cidx(d, 0, ny - 1); cc = cidx(d, 1, ny - 1); }
+  else if (i == nx && j == ny) { ca = cidx(d, nx - 1, ny); cb = cidx(d, nx, ny - 1); cc = cidx(d, nx - 1, ny - 1); }
+  if (ca >= 0)
+    for (int k = 0; k < 11; k++) S[k * N + c] = S[k * N + ca] + S[k * N + cb] - S[k * N + cc];
+  const double sq = sqrt(2.0 / d.gamma);
+  const double u1 = S[c];
+  const double tu = S[1 * N + c] / d.ma * sq, tv = S[2 * N + c] / d.ma * sq, tt = S[3 * N + c];
+  d.U[0 * N + c] = u1;
+  d.U[1 * N + c] = u1 * tu;
+  d.U[2 * N + c] = u1 * tv;
+  d.U[3 * N + c] = u1 * d.cv * tt + 0.5 * u1 * (tu * tu + tv * tv);
+}
+
+// Convert_NS part 2 (src/Hybrid_Sub.f90:236-239): the four whole-array ghost
+// extrapolations, in order.  Single block.
+__global__ void k_convert_ghost(Dev d) {
+  const int nx = d.nx, ny = d.ny, N = d.ncell;
+  double *U = d.U;
+  for (int ph = 0; ph < 4; ph++) {
+    const int len = (ph < 2) ? (ny + 3) : (nx + 3);
+    for (int t = threadIdx.x; t < 4 * len; t += blockDim.x) {
+      const int m = t % 4, r = t / 4 - 1;  // r in [-1, n+1]
+      int cg, c1, c2;
+      if (ph == 0) { cg = cidx(d, 0, r); c1 = cidx(d, 1, r); c2 = cidx(d, 2, r); }
+      else if (ph == 1) { cg = cidx(d, nx, r); c1 = cidx(d, nx - 1, r); c2 = cidx(d, nx - 2, r); }
+      else if (ph == 2) { cg = cidx(d, r, 0); c1 = cidx(d, r, 1); c2 = cidx(d, r, 2); }
+      else { cg = cidx(d, r, ny); c1 = cidx(d, r, ny - 1); c2 = cidx(d, r, ny - 2); }
+      U[m * N + cg] = 2.0 * U[m * N + c1] - U[m * N + c2];
+    }
+    __syncthreads();
+  }
+}
+
+// comput_duvtpckw (src/sub_NS_singlegrid.f90:121-157) over the padded grid,
+// with the T<0.1 stop of interior cells turned into an error flag.
+__global__ void k_prims(Dev d) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.ncell) return;
+  const int N = d.ncell;
+  const double dd = d.U[c], u = d.U[N + c] / dd, v = d.U[2 * N + c] / dd;
+  const double T = (d.U[3 * N + c] - 0.5 * dd * (u * u + v * v)) / (d.cv * dd);
+  d.prim[c] = dd;
+  d.prim[N + c] = u;
+  d.prim[2 * N + c] = v;
+  d.prim[3 * N + c] = T;
+  d.prim[4 * N + c] = (1. / (d.gamma * d.ma * d.ma)) * dd * T;
+  const int i = c % d.NI - 1, j = c / d.NI - 1;
+  if (i >= 1 && i <= d.nx - 1 && j >= 1 && j <= d.ny - 1 && T < 0.10000000149011612) atomicOr(d.err, 1);
+}
+
+// mu = T^0.81/Re with the REAL(4) literal 0.81 (get_viscous, src/sub_Residual.f90:553-576)
+__device__ __forceinline__ double amu_of(const Dev &d, int c) {
+  return 1. / d.re * pow(d.prim[3 * d.ncell + c], 0.8100000023841858);
+}
+
+// Face NS stress/heat flux (Convert_NS src/Hybrid_Sub.f90:257-309,358-407;
+// the same discretisation as Residual_FVM src/sub_Residual.f90:125-165,262-298).
+__device__ void ns_face(const Dev &d, int dir, int i, int j, double out[8]) {
+  const int nx = d.nx, ny = d.ny, N = d.ncell;
+  const double *uu = d.prim + N, *vv = d.prim + 2 * N, *TT = d.prim + 3 * N;
+  double amu1, amk1, dix, diy, djx, djy, diu, div, dit, dju, djv, djt, u1, v1;
+  const double prt = 0.9;
+#define C(ii, jj) cidx(d, ii, jj)
+  if (dir == 0) {
+    if (i == 1) { const double a = amu_of(d, C(i, j)); amu1 = a + 0.; amk1 = d.cp * (a / d.pr + 0. / prt); }
+    else if (i == nx) { const double a = amu_of(d, C(i - 1, j)); amu1 = a + 0.; amk1 = d.cp * (a / d.pr + 0. / prt); }
+    else {
+      const double a0 = amu_of(d, C(i - 1, j)), a1 = amu_of(d, C(i, j));
+      amu1 = (a0 + a1 + 0. + 0.) * 0.5;
+      amk1 = d.cp * ((a0 + a1) / d.pr + (0. + 0.) / prt) * 0.5;
+    }
+    dix = d.x1[C(i, j)] - d.x1[C(i - 1, j)];
+    diy = d.y1[C(i, j)] - d.y1[C(i - 1, j)];
+    djx = (d.x1[C(i - 1, j + 1)] + d.x1[C(i, j + 1)] - d.x1[C(i - 1, j - 1)] - d.x1[C(i, j - 1)]) * 0.25;
+    djy = (d.y1[C(i - 1, j + 1)] + d.y1[C(i, j + 1)] - d.y1[C(i - 1, j - 1)] - d.y1[C(i, j - 1)]) * 0.25;
+    diu = uu[C(i, j)] - uu[C(i - 1, j)];
+    div = vv[C(i, j)] - vv[C(i - 1, j)];
+    dit = TT[C(i, j)] - TT[C(i - 1, j)];
+    dju = (uu[C(i - 1, j + 1)] + uu[C(i, j + 1)] - uu[C(i - 1, j - 1)] - uu[C(i, j - 1)]) * 0.25;
+    djv = (vv[C(i - 1, j + 1)] + vv[C(i, j + 1)] - vv[C(i - 1, j - 1)] - vv[C(i, j - 1)]) * 0.25;
+    djt = (TT[C(i - 1, j + 1)] + TT[C(i, j + 1)] - TT[C(i - 1, j - 1)] - TT[C(i, j - 1)]) * 0.25;
+    u1 = (uu[C(i, j)] + uu[C(i - 1, j)]) * 0.5;
+    v1 = (vv[C(i, j)] + vv[C(i - 1, j)]) * 0.5;
+  } else {
+    if (j == 1) { const double a = amu_of(d, C(i, j)); amu1 = a + 0.; amk1 = d.cp * (a / d.pr + 0. / prt); }
+    else if (j == ny) { const double a = amu_of(d, C(i, j - 1)); amu1 = a + 0.; amk1 = d.cp * (a / d.pr + 0. / prt); }
+    else {
+      const double a0 = amu_of(d, C(i, j)), a1 = amu_of(d, C(i, j - 1));
+      amu1 = (a0 + a1 + 0. + 0.) * 0.5;
+      amk1 = d.cp * ((a1 + a0) / d.pr + (0. + 0.) / prt) * 0.5;
+    }
+    dix = (d.x1[C(i + 1, j - 1)] + d.x1[C(i + 1, j)] - d.x1[C(i - 1, j - 1)] - d.x1[C(i - 1, j)]) * 0.25;
+    diy = (d.y1[C(i + 1, j - 1)] + d.y1[C(i + 1, j)] - d.y1[C(i - 1, j - 1)] - d.y1[C(i - 1, j)]) * 0.25;
+    djx = d.x1[C(i, j)] - d.x1[C(i, j - 1)];
+    djy = d.y1[C(i, j)] - d.y1[C(i, j - 1)];
+    diu = (uu[C(i + 1, j - 1)] + uu[C(i + 1, j)] - uu[C(i - 1, j - 1)] - uu[C(i - 1, j)]) * 0.25;
+    div = (vv[C(i + 1, j - 1)] + vv[C(i + 1, j)] - vv[C(i - 1, j - 1)] - vv[C(i - 1, j)]) * 0.25;
+    dit = (TT[C(i + 1, j - 1)] + TT[C(i + 1, j)] - TT[C(i - 1, j - 1)] - TT[C(i - 1, j)]) * 0.25;
+    dju = uu[C(i, j)] - uu[C(i, j - 1)];
+    djv = vv[C(i, j)] - vv[C(i, j - 1)];
+    djt = TT[C(i, j)] - TT[C(i, j - 1)];
+    u1 = (uu[C(i, j)] + uu[C(i, j - 1)]) * 0.5;
+    v1 = (vv[C(i, j)] + vv[C(i, j - 1)]) * 0.5;
+  }
+#undef C
+  const double ds = 1. / (dix * djy - djx * diy);
+  const double ux = (diu * djy - dju * diy) * ds;
+  const double vx = (div * djy - djv * diy) * ds;
+  const double tx = (dit * djy - djt * diy) * ds;
+  const double uy = (-diu * djx + dju * dix) * ds;
+  const double vy = (-div * djx + djv * dix) * ds;
+  const double ty = (-dit * djx + djt * dix) * ds;
+  out[0] = ((4. / 3.) * ux - (2. / 3.) * vy) * amu1;
+  out[1] = ((4. / 3.) * vy - (2. / 3.) * ux) * amu1;
+  out[2] = (uy + vx) * amu1;
+  out[3] = amu1 * (-(2. / 3.) * ux - (2. / 3.) * vy);
+  out[4] = amk1 * tx;
+  out[5] = amk1 * ty;
+  out[6] = u1;
+  out[7] = v1;
+}
+
+// HoT of Convert_NS (src/Hybrid_Sub.f90:317-349 and :419-448): i-faces
+// (i=1..nx, j=1..ny-1) then j-faces (i=1..nx-1, j=1..ny), one thread each.
+__global__ void k_hot(Dev d) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = d.nx, ny = d.ny, N = d.ncell;
+  const int ni = nx * (ny - 1), nj = (nx - 1) * ny;
+  if (t >= ni + nj) return;
+  const double *S = d.S;
+  const double gm2 = d.gamma * (d.ma * d.ma);
+  // 1.4142135381698608 is sqrt(2.0) evaluated in REAL(4), as the reference does
+  double o[8];
+  if (t < ni) {
+    const int i = t % nx + 1, j = t / nx + 1;
+    ns_face(d, 0, i, j, o);
+    const int c = cidx(d, i, j), cm = cidx(d, i - 1, j);
+    double *tg = d.taoi, *qg = d.qi;
+    if (i != 1 && i != nx) {
+      const double tmp = (S[4 * N + c] + S[5 * N + c] + S[9 * N + c] + S[4 * N + cm] + S[5 * N + cm] + S[9 * N + cm]) / 6.0;
+      tg[0 * N + c] = -((S[4 * N + c] + S[4 * N + cm]) / 2.0 - tmp) / gm2 - o[0];
+      tg[1 * N + c] = -((S[5 * N + c] + S[5 * N + cm]) / 2.0 - tmp) / gm2 - o[1];
+      tg[2 * N + c] = -(S[6 * N + c] + S[6 * N + cm]) / 2.0 / gm2 - o[2];
+      tg[3 * N + c] = -((S[9 * N + c] + S[9 * N + cm]) / 2.0 - tmp) / gm2 - o[3];
+      qg[0 * N + c] = -(S[7 * N + cm] + S[7 * N + c]) / 2.0 * 1.4142135381698608 / ((d.ma * d.ma * d.ma) * pow(d.gamma, 1.5)) - o[4];
+      qg[1 * N + c] = -(S[8 * N + cm] + S[8 * N + c]) / 2.0 * 1.4142135381698608 / ((d.ma * d.ma * d.ma) * pow(d.gamma, 1.5)) - o[5];
+    } else {
+      const int cs = (i == 1) ? c : cm;
+      const double tmp = (S[4 * N + cs] + S[5 * N + cs] + S[9 * N + cs]) / 3.0;
+      tg[0 * N + c] = -(S[4 * N + cs] - tmp) / gm2 - o[0];
+      tg[1 * N + c] = -(S[5 * N + cs] - tmp) / gm2 - o[1];
+      tg[2 * N + c] = -S[6 * N + cs] / gm2 - o[2];
+      tg[3 * N + c] = -(S[9 * N + cs] - tmp) / gm2 - o[3];
+      qg[0 * N + c] = -S[7 * N + cs] * 1.4142135381698608 / ((d.ma * d.ma * d.ma) * pow(d.gamma, 1.5)) - o[4];
+      qg[1 * N + c] = -S[8 * N + cs] * 1.4142135381698608 / ((d.ma * d.ma * d.ma) * pow(d.gamma, 1.5)) - o[5];
+    }
+  } else {
+    const int tt = t - ni;
+    const int i = tt % (nx - 1) + 1, j = tt / (nx - 1) + 1;
+    ns_face(d, 1, i, j, o);
+    const int c = cidx(d, i, j), cm = cidx(d, i, j - 1);
+    double *tg = d.taoj, *qg = d.qj;
+    if (j != 1 && j != ny) {
+      const double tmp = (S[4 * N + c] + S[5 * N + c] + S[9 * N + c] + S[4 * N + cm] + S[5 * N + cm] + S[9 * N + cm]) / 6.0;
+      tg[0 * N + c] = -((S[4 * N + c] + S[4 * N + cm]) / 2.0 - tmp) / gm2 - o[0];
+      tg[1 * N + c] = -((S[5 * N + c] + S[5 * N + cm]) / 2.0 - tmp) / gm2 - o[1];
+      tg[2 * N + c] = -(S[6 * N + c] + S[6 * N + cm]) / 2.0 / gm2 - o[2];
+      tg[3 * N + c] = -((S[9 * N + c] + S[9 * N + cm]) / 2.0 - tmp) / gm2 - o[3];
+      qg[0 * N + c] = -(S[7 * N + cm] + S[7 * N + c]) / 2.0 * 1.4142135381698608 / ((d.ma * d.ma * d.ma) * pow(d.gamma, 1.5)) - o[4];
+      qg[1 * N + c] = -(S[8 * N + cm] + S[8 * N + c]) / 2.0 * 1.4142135381698608 / ((d.ma * d.ma * d.ma) * pow(d.gamma, 1.5)) - o[5];
+    } else {
+      const int cs = (j == 1) ? c : cm;
+      const double tmp = (S[4 * N + cs] + S[5 * N + cs] + S[9 * N + cs]) / 3.0;
+      tg[0 * N + c] = -(S[4 * N + cs] - tmp) / gm2 - o[0];
+      tg[1 * N + c] = -(S[5 * N + cs] - tmp) / gm2 - o[1];
+      tg[2 * N + c] = -S[6 * N + cs] / gm2 - o[2];
+      tg[3 * N + c] = -(S[9 * N + cs] - tmp) / gm2 - o[3];
+      qg[0 * N + c] = -S[7 * N + cs] * 1.4142135381698608 / ((d.ma * d.ma * d.ma) * pow(d.gamma, 1.5)) - o[4];
+      qg[1 * N + c] = -S[8 * N + cs] * 1.4142135381698608 / ((d.ma * d.ma * d.ma) * pow(d.gamma, 1.5)) - o[5];
+    }
+  }
+}
+
+// RMS and max over interior cells of the per-cell RHS moments
+// (DETFP_res, src/GKUA_DETFP_RES.f90:45-54).  Single block.
+__global__ void k_residual(Dev d, double *out) {
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int ncl = (d.nx - 1) * (d.ny - 1);
+  for (int t = threadIdx.x; t < ncl; t += blockDim.x) {
+    const int i = t % (d.nx - 1) + 1, j = t / (d.nx - 1) + 1;
+    const int c = cidx(d, i, j);
+    for (int k = 0; k < 4; k++) {
+      const double r = d.res4[k * d.ncell + c];
+      acc[k] += r * r;
+      acc[4 + k] = fmax(acc[4 + k], fabs(r));
+    }
+  }
+  // sums and maxima reduced separately
+  double s4[4] = {acc[0], acc[1], acc[2], acc[3]};
+  block_sum<4>(s4);
+  __shared__ double shm[4][16];
+  for (int k = 0; k < 4; k++) {
+    double m = acc[4 + k];
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) shm[k][threadIdx.x >> 6] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 4; k++) {
+      double m = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, shm[k][w]);
+      out[k] = sqrt(s4[k] / (1.0 * ncl));
+      out[4 + k] = m;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Layout transposition between the reference's FRD(2,-3:nx+3,-3:ny+3,nvi,nvj)
+// and the device's f[cell][velocity] planes + wall slots.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ size_t frd_idx(const Dev &d, int ird, int i, int j, int v) {
+  return (size_t)ird + 2 * ((size_t)(i + 3) + (size_t)(d.nx + 7) * ((size_t)(j + 3) + (size_t)(d.ny + 7) * v));
+}
+
+template <typename T>
+__global__ void k_frd_in(Dev d, const double *frd, T *FG, T *FH, T *slot) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t ncv = (size_t)d.ncell * d.nv;
+  if (t < ncv) {
+    const int v = (int)(t % d.nv);
+    const int c = (int)(t / d.nv);
+    const int i = c % d.NI - 1, j = c / d.NI - 1;
+    FG[t] = T(frd[frd_idx(d, 0, i, j, v)]);
+    FH[t] = T(frd[frd_idx(d, 1, i, j, v)]);
+    return;
+  }
+  const size_t u = t - ncv;
+  const size_t nsl = (size_t)4 * d.L * d.nv;
+  if (u >= nsl) return;
+  const int v = (int)(u % d.nv);
+  const int pos = (int)((u / d.nv) % d.L);
+  const int face = (int)(u / ((size_t)d.nv * d.L)) + 1;
+  int i, j;
+  if (face == 1) { i = -3; j = pos - 1; }
+  else if (face == 3) { i = d.nx + 3; j = pos - 1; }
+  else if (face == 2) { i = pos - 1; j = -3; }
+  else { i = pos - 1; j = d.ny + 3; }
+  const bool ok = (face == 1 || face == 3) ? (j <= d.ny + 1) : (i <= d.nx + 1);
+  for (int ird = 0; ird < 2; ird++)
+    slot[slot_idx(d, face, pos, ird, v)] = ok ? T(frd[frd_idx(d, ird, i, j, v)]) : T(0);
+}
+
+template <typename T>
+__global__ void k_frd_out(Dev d, double *frd, const T *FG, const T *FH, const T *slot) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t ncv = (size_t)d.ncell * d.nv;
+  if (t < ncv) {
+    const int v = (int)(t % d.nv);
+    const int c = (int)(t / d.nv);
+    const int i = c % d.NI - 1, j = c / d.NI - 1;
+    frd[frd_idx(d, 0, i, j, v)] = (double)FG[t];
+    frd[frd_idx(d, 1, i, j, v)] = (double)FH[t];
+    return;
+  }
+  const size_t u = t - ncv;
+  const size_t nsl = (size_t)4 * d.L * d.nv;
+  if (u >= nsl) return;
+  const int v = (int)(u % d.nv);
+  const int pos = (int)((u / d.nv) % d.L);
+  const int face = (int)(u / ((size_t)d.nv * d.L)) + 1;
+  int i, j;
+  if (face == 1) { i = -3; j = pos - 1; }
+  else if (face == 3) { i = d.nx + 3; j = pos - 1; }
+  else if (face == 2) { i = pos - 1; j = -3; }
+  else { i = pos - 1; j = d.ny + 3; }
+  const bool ok = (face == 1 || face == 3) ? (j <= d.ny + 1) : (i <= d.nx + 1);
+  if (ok)
+    for (int ird = 0; ird < 2; ird++) frd[frd_idx(d, ird, i, j, v)] = (double)slot[slot_idx(d, face, pos, ird, v)];
+}
+
+// ------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------
+template <typename T>
+void launch_wall(const Dev &d, const BcDev &b, T *FG, T *FH, T *slot, hipStream_t s) {
+  const int nn = (b.face == 1 || b.face == 3) ? (b.jend - b.jst) : (b.iend - b.ist);
+  if (nn <= 0) return;
+  hipLaunchKernelGGL(k_wall<T>, dim3(nn), dim3(256), 0, s, d, b, FG, FH, slot);
+}
+template <typename T>
+void launch_farfield(const Dev &d, const BcDev &b, T *FG, T *FH, hipStream_t s) {
+  const int nn = (b.face == 1 || b.face == 3) ? (b.jend - b.jst + 1) : (b.iend - b.ist + 1);
+  hipLaunchKernelGGL(k_farfield<T>, dim3(2 * nn), dim3(256), 0, s, d, b, FG, FH);
+}
+template <typename T>
+void launch_periodic(const Dev &d, const BcDev &b, T *FG, T *FH, hipStream_t s) {
+  const int nn = (b.face == 1 || b.face == 3) ? (b.jend - b.jst + 1) : (b.iend - b.ist + 1);
+  hipLaunchKernelGGL(k_periodic<T>, dim3(2 * nn), dim3(256), 0, s, d, b, FG, FH);
+}
+template <typename T>
+void launch_rhs(const Dev &d, const T *FG, const T *FH, const T *slot, T *RG, T *RH, hipStream_t s) {
+  hipLaunchKernelGGL(k_rhs<T>, dim3((d.nx - 1) * (d.ny - 1)), dim3(256), 0, s, d, FG, FH, slot, RG, RH);
+}
+template <typename T>
+int sweep_vt(const Dev &d) {
+  // largest power-of-two tile (<=64) whose 3-plane LDS ring fits in 64 KiB
+  int vt = 64;
+  while (vt > 1 && (size_t)3 * (d.nx + 1) * vt * 2 * sizeof(T) > 64 * 1024) vt >>= 1;
+  return vt;
+}
+template <typename T>
+void launch_sweep(const Dev &d, T *FG, T *FH, const T *RG, const T *RH, hipStream_t s) {
+  const int vt = sweep_vt<T>(d);
+  const size_t lds = (size_t)3 * (d.nx + 1) * vt * 2 * sizeof(T);
+  const int nb = (d.nv + vt - 1) / vt;
+  hipLaunchKernelGGL(k_sweep<T>, dim3(nb), dim3(256), lds, s, d, vt, FG, FH, RG, RH);
+}
+template <typename T>
+void launch_moments(const Dev &d, T *FG, T *FH, hipStream_t s) {
+  hipLaunchKernelGGL(k_moments<T>, dim3(d.ncell), dim3(256), 0, s, d, FG, FH);
+}
+template <typename T>
+void launch_frd_in(const Dev &d, const double *frd, T *FG, T *FH, T *slot, hipStream_t s) {
+  const size_t n = (size_t)d.ncell * d.nv + (size_t)4 * d.L * d.nv;
+  hipLaunchKernelGGL(k_frd_in<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, frd, FG, FH, slot);
+}
+template <typename T>
+void launch_frd_out(const Dev &d, double *frd, const T *FG, const T *FH, const T *slot, hipStream_t s) {
+  const size_t n = (size_t)d.ncell * d.nv + (size_t)4 * d.L * d.nv;
+  hipLaunchKernelGGL(k_frd_out<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, frd, FG, FH, slot);
+}
+
+void launch_save_s0(const Dev &d, hipStream_t s) {
+  const int n = (d.nx - 1) * (d.ny - 1);
+  hipLaunchKernelGGL(k_save_s0, dim3((n + 255) / 256), dim3(256), 0, s, d);
+}
+void launch_det_wall(const Dev &d, const int *list, int n, double *swall, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_det_wall, dim3((n + 127) / 128), dim3(128), 0, s, d, list, n, swall);
+}
+void launch_convert(const Dev &d, hipStream_t s) {
+  const int n = (d.nx + 1) * (d.ny + 1);
+  hipLaunchKernelGGL(k_convert_u, dim3((n + 255) / 256), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_convert_ghost, dim3(1), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_prims, dim3((d.ncell + 255) / 256), dim3(256), 0, s, d);
+}
+void launch_hot(const Dev &d, hipStream_t s) {
+  const int n = d.nx * (d.ny - 1) + (d.nx - 1) * d.ny;
+  hipLaunchKernelGGL(k_hot, dim3((n + 127) / 128), dim3(128), 0, s, d);
+}
+void launch_residual(const Dev &d, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_residual, dim3(1), dim3(1024), 0, s, d, out);
+}
+
+#define INST(T)                                                                                   \
+  template void launch_wall<T>(const Dev &, const BcDev &, T *, T *, T *, hipStream_t);           \
+  template void launch_farfield<T>(const Dev &, const BcDev &, T *, T *, hipStream_t);            \
+  template void launch_periodic<T>(const Dev &, const BcDev &, T *, T *, hipStream_t);            \
+  template void launch_rhs<T>(const Dev &, const T *, const T *, const T *, T *, T *, hipStream_t); \
+  template void launch_sweep<T>(const Dev &, T *, T *, const T *, const T *, hipStream_t);        \
+  template void launch_moments<T>(const Dev &, T *, T *, hipStream_t);                            \
+  template void launch_frd_in<T>(const Dev &, const double *, T *, T *, T *, hipStream_t);        \
+  template void launch_frd_out<T>(const Dev &, double *, const T *, const T *, const T *, hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace gsis

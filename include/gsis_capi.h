@@ -1,0 +1,164 @@
+/*
+ * gsis_capi.h -- C-ABI of the MI355X-native GSIS kinetic loop.
+ *
+ * The library replaces the body of one outer GSIS iteration of the Fortran
+ * code Hybrid2D (subroutine Hybrid_Time_advance_LU_SGS, src/Hybrid_Sub.f90:4-160,
+ * called from src/opencfd_ec2d.f90:46) and the residual monitor DETFP_res
+ * (src/GKUA_DETFP_RES.f90:4-60, called from src/opencfd_ec2d.f90:47).
+ *
+ * The reference keeps all state in Fortran module globals (src/COMMONDATA.f90:44-170);
+ * here every array crosses the boundary explicitly, as plain pointers in the
+ * reference's own Fortran layout (column-major, ghost lower bounds as noted),
+ * so that a Fortran host passes its allocatable arrays unchanged through
+ * ISO_C_BINDING.  The library copies in during create/upload and copies out
+ * during download; it never keeps a host pointer past a call.
+ *
+ * Every entry point returns 0 on success or one of the GSIS_E* codes; the
+ * message of the last failure is returned by gsis_last_error().  No C++
+ * exception crosses this boundary.  A handle is single-threaded and
+ * non-re-entrant.
+ */
+#ifndef GSIS_CAPI_H
+#define GSIS_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define GSIS_OK 0
+#define GSIS_EARG 1         /* bad argument / unsupported size */
+#define GSIS_EHIP 2         /* HIP runtime error */
+#define GSIS_ENAN 3         /* NaN in the residual (reference: src/IO.f90:236-244) */
+#define GSIS_EUNSUPPORTED 4 /* option not implemented */
+#define GSIS_ETEMP 5        /* T < 0.1 in an interior cell (src/sub_NS_singlegrid.f90:148-151) */
+
+/* boundary codes, as stored after negation in read_bcin (src/sub_init.f90:278,
+ * constants src/COMMONDATA.f90:14) */
+#define GSIS_BC_WALL (-2)
+#define GSIS_BC_FARFIELD (-4)
+#define GSIS_BC_PERIODIC (-5) /* "inlet": periodic copy, src/GKUA_boundary.f90:219-273 */
+
+/* kinetic flux schemes (src/COMMONDATA.f90:18) */
+#define GSIS_SCHEME_UP1 0
+#define GSIS_SCHEME_NND 1
+
+/* step modes */
+#define GSIS_MODE_M0 0 /* the step as shipped: DVM implicit step + HoT, synthetic solve off */
+#define GSIS_MODE_M1 1 /* synthetic NS solve enabled (src/Hybrid_Sub.f90:45-153) */
+
+/* arithmetic of the distribution sweeps */
+#define GSIS_PREC_FP64 0
+#define GSIS_PREC_FP32 1 /* FP32 storage + sweeps, FP64 moments/macro/synthetic */
+
+/* One boundary subface, mirrors BC_MSG_TYPE (src/COMMONDATA.f90:34-41) as
+ * filled by read_bcin (src/sub_init.f90:266-278).  Node indices are 1-based. */
+typedef struct gsis_bc {
+  int32_t face;  /* 1 i-min, 2 j-min, 3 i-max, 4 j-max (src/sub_init.f90:269-277) */
+  int32_t ist, iend, jst, jend;
+  int32_t code;  /* GSIS_BC_* */
+  double uw, vw; /* diffuse-wall velocity in sqrt(2RT) units; the reference
+                    hard-codes 0 (src/GKUA_boundary.f90:66-67) */
+} gsis_bc;
+
+/* Everything the reference reads from module globals during one step. */
+typedef struct gsis_params {
+  int32_t nx, ny;   /* node counts; cells i=1..nx-1, j=1..ny-1 */
+  int32_t nvi, nvj; /* discrete velocities per direction (NVIt, NVJt) */
+  const double *vi, *vj; /* nodes B%vi(1:nvi), B%vj(1:nvj) */
+  /* effective 1-D weights: the 2-D quadrature weight of (iv,jv) is wi[iv]*wj[jv].
+     Gauss-Legendre: wi = FBAVX(ivp)*wi_GL (src/GKUA_DETFP.f90:314);
+     Gauss-Hermite: wi = wi_GH*exp(vi^2) (src/GKUA_DETFP.f90:200-202). */
+  const double *wi, *wj;
+  /* node coordinates with LAP=2 ghost layers, Fortran x(1-2:nx+2,1-2:ny+2),
+     as produced by Update_coordinate_buffer_onemesh (src/sub_init.f90:719-823) */
+  const double *x, *y;
+  /* geometry, Fortran (nx,ny) arrays from Creat_Mesh (src/sub_init.f90:183-208) */
+  const double *vol, *si, *sj, *ni1, *ni2, *nj1, *nj2;
+  int32_t nbc;
+  const gsis_bc *bc; /* in the order of generic.inp (the order the BCs are applied) */
+  /* kinetic constants (src/sub_init.f90:14,65-66; src/sub_init.f90:402) */
+  double cfc, x1s, cpr, twall, dt;
+  /* NS-side constants (src/sub_init.f90:16,32-35; src/opencfd_ec2d.f90:134) */
+  double ma, gamma, re, pr, cv, cp, aoa;
+  int32_t scheme;     /* GSIS_SCHEME_* (Iflag_Scheme, src/COMMONDATA.f90:18) */
+  int32_t mode;       /* GSIS_MODE_* */
+  int32_t precision;  /* GSIS_PREC_* */
+  int32_t if_viscous; /* If_viscous (src/opencfd_ec2d.f90:137) */
+  /* synthetic NS solve (mode M1): local time step and inner-loop control
+     (src/sub_NS_singlegrid.f90:160-183, src/Hybrid_Sub.f90:99-104) */
+  int32_t local_dt;
+  double cfl, dtmax, dtmin;
+  double ns_tol;    /* inner loop runs while max |Res_4| > ns_tol (reference 1e-6) */
+  int32_t ns_maxit; /* safety cap on inner iterations (0 = 100000) */
+} gsis_params;
+
+typedef struct gsis_handle gsis_handle;
+
+/* Create the device state.  Replaces the allocation part of Alloc_GKUA_FRD
+ * (src/sub_init.f90:329-386); the state starts zeroed like the reference's. */
+int gsis_create(const gsis_params *p, gsis_handle **out);
+
+/* Upload B%FRD, Fortran FRD(2,-3:nx+3,-3:ny+3,nvi,nvj).  Used after the
+ * initial Maxwellian fill (src/GKUA_sudufenbu.f90:22-46) and on restart
+ * (src/Backup_write_read.f90:93-109). */
+int gsis_upload_frd(gsis_handle *h, const double *frd);
+
+/* Upload S_GKUA(11,-3:nx+3,-3:ny+3), CF(-1:nx+1,-1:ny+1), U(4,-1:nx+1,-1:ny+1).
+ * u0 (the synthetic solver's state B%U0, same shape as U) may be NULL; the
+ * reference initialises it to (1,0,0,1) (src/sub_init.f90:159). */
+int gsis_upload_macro(gsis_handle *h, const double *s, const double *cf, const double *u,
+                      const double *u0);
+
+/* Kinetic BCs + moments of the current distribution, the pair that
+ * GKUA_sudufenbu runs after the Maxwellian fill (src/GKUA_sudufenbu.f90:43,53):
+ * GKUA_Phy_boundary then DETFP (with DETFPwall). */
+int gsis_refresh_moments(gsis_handle *h);
+
+/* nsteps outer iterations of Hybrid_Time_advance_LU_SGS (src/Hybrid_Sub.f90:4-160).
+ * Asynchronous: returns once the work is queued on the handle's stream. */
+int gsis_step(gsis_handle *h, int32_t nsteps);
+
+/* RMS of the four conserved moments of the last step's kinetic RHS,
+ * B%Res_rms(1:4) of DETFP_res (src/GKUA_DETFP_RES.f90:45-54).
+ * Returns GSIS_ENAN when res_rms[0] is NaN (src/IO.f90:236-244). */
+int gsis_residual(gsis_handle *h, double res_rms[4]);
+
+/* Download macro state.  Any pointer may be NULL.  Shapes (Fortran):
+ *   s, s0   : (11,-3:nx+3,-3:ny+3)     S_GKUA, S0_GKUA
+ *   cf      : (-1:nx+1,-1:ny+1)
+ *   u       : (4,-1:nx+1,-1:ny+1)
+ *   taonsi_g, taonsj_g : (4,-1:nx+1,-1:ny+1)   qi_g, qj_g : (2,-1:nx+1,-1:ny+1)
+ *   swall   : for each wall/farfield BC in list order, (10, nk) with nk the
+ *             BC's node count along the face (BC%Swall(1:10,...)). */
+int gsis_download_macro(gsis_handle *h, double *s, double *s0, double *cf, double *u,
+                        double *taonsi_g, double *taonsj_g, double *qi_g, double *qj_g,
+                        double *swall);
+
+/* Download B%FRD (same layout as gsis_upload_frd), for backup_Mesco_write
+ * (src/Backup_write_read.f90:150-209). */
+int gsis_download_frd(gsis_handle *h, double *frd);
+
+/* Step counter and accumulated time (Mesh(1)%Kstep, Mesh(1)%tt,
+ * src/Hybrid_Sub.f90:156-157) and, in mode M1, the last inner iteration count
+ * (MP%iter). */
+int gsis_counters(gsis_handle *h, int32_t *kstep, double *tt, int32_t *ns_iter);
+
+/* Kernel timing with HIP events on the handle's stream.  No counterpart in
+ * the reference: it exists for benchmarks.  enable=1 clears and starts the
+ * accumulators, 0 clears and stops them, -1 only reads.  ms[0..2] receive the
+ * summed durations (ms) of the kinetic RHS kernel, the LU-SGS sweep kernel and
+ * the moments kernel over *nsteps timed steps. */
+int gsis_timing(gsis_handle *h, int32_t enable, double *ms, int32_t *nsteps);
+
+int gsis_destroy(gsis_handle *h);
+
+const char *gsis_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSIS_CAPI_H */

@@ -1,0 +1,177 @@
+"""ctypes wrapper of the CPU oracle (gsis_oracle.c).  TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as
+the checker; the product path (the GsisSolver over libgsis.so) never imports
+it.  OracleSolver offers the same methods as the product's GsisSolver so that
+both can be driven from the same inputs.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB = HERE / "libgsis_oracle.so"
+SOURCES = [HERE / "gsis_oracle.c", HERE / "gsis_oracle_ns.c"]
+
+
+def build(force: bool = False) -> pathlib.Path:
+    """Compile the oracle with gcc (FP contraction off, like the reference's
+    separate multiply/add)."""
+    if force or not LIB.exists() or any(s.stat().st_mtime > LIB.stat().st_mtime for s in SOURCES):
+        cmd = ["gcc", "-O2", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-o", str(LIB)]
+        cmd += [str(s) for s in SOURCES] + ["-lm"]
+        subprocess.run(cmd, check=True)
+    return LIB
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        build()
+        lib = ctypes.CDLL(str(LIB))
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        lib.orc_create.restype = vp
+        lib.orc_create.argtypes = [ctypes.c_void_p]
+        lib.orc_destroy.argtypes = [vp]
+        lib.orc_frd_len.restype = ctypes.c_size_t
+        lib.orc_frd_len.argtypes = [vp]
+        lib.orc_set_frd.argtypes = [vp, dp]
+        lib.orc_get_frd.argtypes = [vp, dp]
+        lib.orc_get_frd00.argtypes = [vp, dp]
+        lib.orc_set_macro.argtypes = [vp, dp, dp, dp, dp]
+        lib.orc_get_macro.argtypes = [vp] + [dp] * 9
+        lib.orc_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int)]
+        lib.orc_phy_boundary.argtypes = [vp]
+        lib.orc_detfp.argtypes = [vp]
+        lib.orc_convert_ns.argtypes = [vp]
+        lib.orc_convert_ns.restype = ctypes.c_int
+        lib.orc_step.argtypes = [vp, ctypes.c_int]
+        lib.orc_step.restype = ctypes.c_int
+        lib.orc_residual.argtypes = [vp, dp, dp]
+        lib.orc_residual.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class OracleSolver:
+    """CPU restatement of one GSIS case (same interface as GsisSolver)."""
+
+    def __init__(self, case, mode: int = 0, **kw):
+        from importlib import import_module
+
+        self.case = case
+        pkg = import_module(type(case).__module__.rsplit(".", 1)[0])
+        self.cases = import_module(pkg.__name__ + ".cases")
+        self.pack = self.cases.ParamPack(case, precision=0, mode=mode, **kw)
+        self.mode = mode
+        self.lib = load()
+        self.h = self.lib.orc_create(ctypes.addressof(self.pack.p))
+
+    def upload(self, frd=None, s=None, cf=None, u=None, u0=None):
+        if frd is not None:
+            frd = np.asfortranarray(frd, dtype=np.float64)
+            self.lib.orc_set_frd(self.h, _dp(frd))
+        arrs = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (s, cf, u, u0)]
+        if any(a is not None for a in arrs):
+            self.lib.orc_set_macro(self.h, *[None if a is None else _dp(a) for a in arrs])
+
+    def refresh_moments(self):
+        self.lib.orc_phy_boundary(self.h)
+        self.lib.orc_detfp(self.h)
+
+    def initialize(self):
+        frd, s, cf, u = self.cases.initial_state(self.case)
+        self.upload(frd=frd, s=s, cf=cf, u=u, u0=u if self.mode == 1 else None)
+        del frd
+        self.refresh_moments()
+        m = self.macro()
+        u = self.cases.ns_boundary_init(self.case, m["u"], m["swall"])
+        self.upload(u=u, u0=u if self.mode == 1 else None)
+
+    def macro(self) -> dict:
+        c = self.case
+        nx, ny = c.nx, c.ny
+        out = {
+            "s": np.zeros((11, nx + 7, ny + 7), order="F"),
+            "s0": np.zeros((11, nx + 7, ny + 7), order="F"),
+            "cf": np.zeros((nx + 3, ny + 3), order="F"),
+            "u": np.zeros((4, nx + 3, ny + 3), order="F"),
+            "taoi": np.zeros((4, nx + 3, ny + 3), order="F"),
+            "taoj": np.zeros((4, nx + 3, ny + 3), order="F"),
+            "qi": np.zeros((2, nx + 3, ny + 3), order="F"),
+            "qj": np.zeros((2, nx + 3, ny + 3), order="F"),
+            "swall": np.zeros(max(1, self.cases.swall_len(c))),
+        }
+        keys = ("s", "s0", "cf", "u", "taoi", "taoj", "qi", "qj", "swall")
+        self.lib.orc_get_macro(self.h, *[_dp(out[k]) for k in keys])
+        return out
+
+    def frd(self) -> np.ndarray:
+        a = np.zeros(self.cases.frd_shape(self.case), order="F")
+        self.lib.orc_get_frd(self.h, _dp(a))
+        return a
+
+    def frd00(self) -> np.ndarray:
+        a = np.zeros(self.cases.frd_shape(self.case), order="F")
+        self.lib.orc_get_frd00(self.h, _dp(a))
+        return a
+
+    def step(self, n: int = 1):
+        rc = self.lib.orc_step(self.h, int(n))
+        if rc:
+            raise OracleError(f"oracle step failed with code {rc}")
+
+    def residual(self, with_max: bool = False):
+        r = np.zeros(4)
+        m = np.zeros(4)
+        rc = self.lib.orc_residual(self.h, _dp(r), _dp(m))
+        if rc:
+            raise OracleError(f"oracle residual code {rc}")
+        return (r, m) if with_max else r
+
+    def counters(self):
+        k = ctypes.c_int()
+        t = ctypes.c_double()
+        it = ctypes.c_int()
+        self.lib.orc_counters(self.h, ctypes.byref(k), ctypes.byref(t), ctypes.byref(it))
+        return k.value, t.value, it.value
+
+    def run(self, max_steps: int, tol: float = 1e-8, show: int = 50):
+        hist = []
+        done = 0
+        while done < max_steps:
+            n = min(show, max_steps - done)
+            self.step(n)
+            done += n
+            r = self.residual()
+            hist.append((done, r.copy()))
+            if np.all(r < tol):
+                break
+        return done, hist
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,39 @@
+"""The C-ABI library loads and exports every entry point include/gsis_capi.h
+declares; argument errors are reported without touching a GPU."""
+import ctypes
+import re
+
+from helpers import ROOT, load_pkg
+
+
+def test_header_and_library_agree():
+    g = load_pkg()
+    hdr = (ROOT / "include" / "gsis_capi.h").read_text()
+    declared = set(re.findall(r"\b(gsis_[a-z_]+)\s*\(", hdr))
+    assert declared == set(g.capi.EXPORTED)
+    lib = g.capi.load_library()
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_null_params_rejected_without_gpu():
+    g = load_pkg()
+    lib = g.capi.load_library()
+    h = ctypes.c_void_p()
+    rc = lib.gsis_create(None, ctypes.byref(h))
+    assert rc == g.capi.GSIS_EARG
+    assert b"NULL" in lib.gsis_last_error()
+
+
+def test_small_mesh_rejected_without_gpu():
+    g = load_pkg()
+    lib = g.capi.load_library()
+    case = g.cases.cavity_case(8, 8, kn=0.1)
+    pack = g.cases.ParamPack(case)
+    pack.p.nx = 4
+    h = ctypes.c_void_p()
+    assert lib.gsis_create(ctypes.byref(pack.p), ctypes.byref(h)) == g.capi.GSIS_EARG
+    pack.p.nx = case.nx
+    pack.bc_arr[0].face = 3  # face index does not match the range
+    assert lib.gsis_create(ctypes.byref(pack.p), ctypes.byref(h)) == g.capi.GSIS_EARG
+    assert b"face" in lib.gsis_last_error()
