@@ -164,6 +164,20 @@ size_t orc_frd_len(const orc_t *o) { return FPLANE * (size_t)o->nvi * o->nvj; }
 void orc_set_frd(orc_t *o, const double *frd) { memcpy(o->F, frd, orc_frd_len(o) * sizeof(double)); }
 void orc_get_frd(const orc_t *o, double *frd) { memcpy(frd, o->F, orc_frd_len(o) * sizeof(double)); }
 void orc_get_frd00(const orc_t *o, double *frd) { memcpy(frd, o->F00, orc_frd_len(o) * sizeof(double)); }
+void orc_get_frd0(const orc_t *o, double *frd) { memcpy(frd, o->F0, orc_frd_len(o) * sizeof(double)); }
+/* primitives and viscosity of the last comput_duvtpckw/get_viscous:
+   out[5][(nx+3)(ny+3)] = d, u, v, T, mu on (-1:nx+1,-1:ny+1) */
+void orc_get_prims(const orc_t *o, double *out) {
+  size_t nc = (size_t)NI3 * (o->ny + 3);
+  memcpy(out, o->d, nc * sizeof(double));
+  memcpy(out + nc, o->uu, nc * sizeof(double));
+  memcpy(out + 2 * nc, o->vv, nc * sizeof(double));
+  memcpy(out + 3 * nc, o->T, nc * sizeof(double));
+  memcpy(out + 4 * nc, o->amu, nc * sizeof(double));
+}
+/* Get_Feq on the current S (src/GKUA_Residual.f90:420-469) into FRD0 */
+static void get_feq(orc_t *o);
+void orc_feq(orc_t *o) { get_feq(o); }
 
 void orc_set_macro(orc_t *o, const double *s, const double *cf, const double *u, const double *u0) {
   size_t nc = (size_t)NI3 * (o->ny + 3);

@@ -19,6 +19,9 @@ size_t orc_frd_len(const orc_t *o);
 void orc_set_frd(orc_t *o, const double *frd);
 void orc_get_frd(const orc_t *o, double *frd);
 void orc_get_frd00(const orc_t *o, double *frd);
+void orc_get_frd0(const orc_t *o, double *frd);
+void orc_get_prims(const orc_t *o, double *out);
+void orc_feq(orc_t *o);
 void orc_set_macro(orc_t *o, const double *s, const double *cf, const double *u, const double *u0);
 void orc_get_macro(const orc_t *o, double *s, double *s0, double *cf, double *u, double *taonsi_g,
                    double *taonsj_g, double *qi_g, double *qj_g, double *swall);

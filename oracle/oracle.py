@@ -46,6 +46,9 @@ def load():
         lib.orc_set_frd.argtypes = [vp, dp]
         lib.orc_get_frd.argtypes = [vp, dp]
         lib.orc_get_frd00.argtypes = [vp, dp]
+        lib.orc_get_frd0.argtypes = [vp, dp]
+        lib.orc_get_prims.argtypes = [vp, dp]
+        lib.orc_feq.argtypes = [vp]
         lib.orc_set_macro.argtypes = [vp, dp, dp, dp, dp]
         lib.orc_get_macro.argtypes = [vp] + [dp] * 9
         lib.orc_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int)]
@@ -131,6 +134,22 @@ class OracleSolver:
         a = np.zeros(self.cases.frd_shape(self.case), order="F")
         self.lib.orc_get_frd00(self.h, _dp(a))
         return a
+
+    def feq(self) -> np.ndarray:
+        """Shakhov equilibrium of the current S on interior cells (FRD0)."""
+        self.lib.orc_feq(self.h)
+        a = np.zeros(self.cases.frd_shape(self.case), order="F")
+        self.lib.orc_get_frd0(self.h, _dp(a))
+        return a
+
+    def prims(self) -> np.ndarray:
+        c = self.case
+        a = np.zeros((5, c.nx + 3, c.ny + 3), order="F")
+        self.lib.orc_get_prims(self.h, _dp(a))
+        return a
+
+    def convert_ns(self) -> int:
+        return self.lib.orc_convert_ns(self.h)
 
     def step(self, n: int = 1):
         rc = self.lib.orc_step(self.h, int(n))

@@ -1,0 +1,225 @@
+"""Checks that pin each part of the oracle's step by what the mathematics fixes,
+independently of how the oracle computes it:
+
+* LU-SGS sweep == (I + D^-1 U)(D - L)^-1 (vol * RHS), solved densely with numpy
+  (GKUA_Residual.f90:34-109: forward recursion, non-recursive backward pass);
+* NND upwind flux is exact on linear data (RHSFDVsub, GKUA_Residual.f90:145-320);
+* the diffuse wall carries zero net mass flux (NWGaussLegendre + GKUA_wall_boundary);
+* a gas at rest at the wall temperature is a fixed point of the whole step;
+* the reduced Shakhov equilibrium keeps n, nu, energy and has heat flux (1-Pr) q
+  (Get_Feq, GKUA_Residual.f90:420-469; derived in 2-D polar moments);
+* Convert_NS's face stress and heat flux are exact on linear fields
+  (Hybrid_Sub.f90:358-407).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import load_oracle, load_pkg
+
+F32_CLAMP = float(np.float32(1e-30))
+
+
+def _case_periodic_box(n=10, nv=8, cfc=1e-14):
+    g = load_pkg().cases
+    s = np.linspace(0.0, 1.0, n + 1)
+    xn, yn = np.meshgrid(s, s, indexing="ij")
+    nx = ny = n + 1
+    P = -5
+    bcs = [g.BC(2, 1, nx, 1, 1, P), g.BC(3, nx, nx, 1, ny, P), g.BC(4, 1, nx, ny, ny, P), g.BC(1, 1, 1, 1, ny, P)]
+    mesh = g.build_mesh(xn, yn, bcs)
+    v, w = g.sinh_velocity_grid(nv, 6.0)
+    const = g.constants_from_kn(1.0, dt=0.5)
+    const["cfc"] = cfc
+    return g.Case("box", mesh, bcs, v, w, v.copy(), w.copy(), const, "rest")
+
+
+def test_sweep_equals_dense_lusgs():
+    g = load_pkg()
+    case = g.cases.cavity_case(6, 8, kn=0.1, lid=0.3, dt=0.7)
+    o = load_oracle().OracleSolver(case)
+    o.initialize()
+    o.step(3)
+    cf = o.macro()["cf"]
+    F0 = o.frd()
+    o.step(1)
+    F1 = o.frd()
+    R = o.frd00()
+    m, nx, ny, dt = case.mesh, case.nx, case.ny, case.const["dt"]
+    cells = [(i, j) for j in range(1, ny) for i in range(1, nx)]
+    idx = {c: k for k, c in enumerate(cells)}
+    worst = 0.0
+    for jv in range(case.nvj):
+        for iv in range(case.nvi):
+            vx, vy = case.vi[iv], case.vj[jv]
+            ax = lambda i, j: (vx * m.ni1[i - 1, j - 1] + vy * m.ni2[i - 1, j - 1]) * m.si[i - 1, j - 1]
+            ay = lambda i, j: (vx * m.nj1[i - 1, j - 1] + vy * m.nj2[i - 1, j - 1]) * m.sj[i - 1, j - 1]
+            n = len(cells)
+            M = np.zeros((n, n))
+            N = np.zeros((n, n))
+            alfa = np.zeros(n)
+            for (i, j), k in idx.items():
+                vol = m.vol[i - 1, j - 1]
+                alfa[k] = vol / dt + (max(ax(i + 1, j), 0) - min(ax(i, j), 0) + max(ay(i, j + 1), 0)
+                                      - min(ay(i, j), 0)) + vol * cf[i + 1, j + 1]
+                M[k, k] = alfa[k]
+                if i > 1:
+                    M[k, idx[(i - 1, j)]] = -max(ax(i, j), 0)
+                if j > 1:
+                    M[k, idx[(i, j - 1)]] = -max(ay(i, j), 0)
+                if i < nx - 1:
+                    N[k, idx[(i + 1, j)]] = -min(ax(i + 1, j), 0)
+                if j < ny - 1:
+                    N[k, idx[(i, j + 1)]] = -min(ay(i, j + 1), 0)
+            for r in range(2):
+                b = np.array([m.vol[i - 1, j - 1] * R[r, i + 3, j + 3, iv, jv] for (i, j) in cells])
+                dstar = np.linalg.solve(M, b)
+                d = dstar + (N @ dstar) / alfa
+                f0 = np.array([F0[r, i + 3, j + 3, iv, jv] for (i, j) in cells])
+                f1 = np.array([F1[r, i + 3, j + 3, iv, jv] for (i, j) in cells])
+                want = np.maximum(f0 + d, F32_CLAMP)
+                worst = max(worst, np.max(np.abs(f1 - want)) / np.max(np.abs(want)))
+    assert worst < 1e-12, worst
+
+
+def test_nnd_flux_exact_on_linear_data():
+    case = _case_periodic_box()
+    o = load_oracle().OracleSolver(case)
+    o.initialize()
+    F = o.frd()
+    ex, ey = 0.01, -0.02
+    ii = np.arange(-3, case.nx + 4)[:, None]
+    jj = np.arange(-3, case.ny + 4)[None, :]
+    lin = 1.0 + ex * ii + ey * jj
+    F = F * lin[None, :, :, None, None]
+    o.upload(frd=F)
+    o.step(1)
+    R = o.frd00()
+    n = case.nx - 1
+    VX = case.vi[:, None]
+    VY = case.vj[None, :]
+    worst = 0.0
+    for i in range(3, case.nx - 2):
+        for j in range(3, case.ny - 2):
+            for r in range(2):
+                g0 = F[r, i + 3, j + 3] / lin[i + 3, j + 3]
+                want = -n * g0 * (VX * ex + VY * ey)
+                got = R[r, i + 3, j + 3]
+                worst = max(worst, np.max(np.abs(got - want)) / np.max(np.abs(want)))
+    assert worst < 1e-9, worst
+
+
+def test_wall_zero_mass_flux():
+    g = load_pkg()
+    case = g.cases.cavity_case(8, 16, kn=0.1, lid=0.2, dt=0.5)
+    o = load_oracle().OracleSolver(case)
+    o.initialize()
+    o.step(2)
+    F = o.frd()
+    m = case.mesh
+    W = np.outer(case.wi, case.wj)
+    VX, VY = np.meshgrid(case.vi, case.vj, indexing="ij")
+    nx, ny = case.nx, case.ny
+    checked = 0
+    for b in case.bcs:
+        for k in range((b.jend - b.jst) if b.face in (1, 3) else (b.iend - b.ist)):
+            if b.face == 1:
+                j = b.jst + k; sl = F[0, 0, j + 3]; n1, n2 = m.ni1[0, j - 1], m.ni2[0, j - 1]
+            elif b.face == 3:
+                j = b.jst + k; sl = F[0, nx + 6, j + 3]; n1, n2 = m.ni1[nx - 1, j - 1], m.ni2[nx - 1, j - 1]
+            elif b.face == 2:
+                i = b.ist + k; sl = F[0, i + 3, 0]; n1, n2 = m.nj1[i - 1, 0], m.nj2[i - 1, 0]
+            else:
+                i = b.ist + k; sl = F[0, i + 3, ny + 6]; n1, n2 = m.nj1[i - 1, ny - 1], m.nj2[i - 1, ny - 1]
+            vn = VX * n1 + VY * n2
+            flux = np.sum(W * vn * sl)
+            scale = np.sum(W * np.abs(vn) * sl)
+            assert abs(flux) < 1e-13 * scale, (b.face, k, flux)
+            checked += 1
+    assert checked == 4 * 8
+
+
+def test_rest_state_is_fixed_point():
+    g = load_pkg()
+    case = g.cases.cavity_case(4, 64, kn=0.5, lid=0.0, dt=1.0)
+    o = load_oracle().OracleSolver(case)
+    o.initialize()
+    F0 = o.frd()
+    o.step(3)
+    F1 = o.frd()
+    sl = (slice(None), slice(4, case.nx + 3), slice(4, case.ny + 3))
+    # drift allowed: quadrature error (~1e-12 on this grid) times the collision frequency
+    assert np.max(np.abs(F1[sl] - F0[sl])) < 1e-10 * np.max(F0[sl])
+    assert np.max(o.residual()) < 1e-10
+
+
+def test_shakhov_moments_and_heat_flux():
+    g = load_pkg()
+    case = g.cases.cavity_case(4, 96, kn=0.1, dt=1.0)
+    o = load_oracle().OracleSolver(case)
+    o.initialize()
+    m = o.macro()
+    s = m["s"].copy()
+    n0, u0, v0, T0, qx0, qy0 = 1.1, 0.2, -0.1, 1.2, 0.05, -0.03
+    s[0], s[1], s[2], s[3], s[7], s[8] = n0, u0, v0, T0, qx0, qy0
+    o.upload(s=s)
+    fe = o.feq()[:, 5, 5]  # cell (2,2)
+    G, H = fe[0], fe[1]
+    W = np.outer(case.wi, case.wj)
+    VX, VY = np.meshgrid(case.vi, case.vj, indexing="ij")
+    cx, cy = VX - u0, VY - v0
+    c2 = cx * cx + cy * cy
+    n = np.sum(W * G)
+    assert abs(n - n0) < 1e-10
+    assert abs(np.sum(W * VX * G) - n0 * u0) < 1e-10
+    assert abs(np.sum(W * VY * G) - n0 * v0) < 1e-10
+    # energy moment S4 = sum(v^2 G + H) = 1.5 n T + n |u|^2 (DETFP, src/GKUA_DETFP.f90:71,330)
+    assert abs(np.sum(W * (VX ** 2 * G + VY ** 2 * G + H)) - (1.5 * n0 * T0 + n0 * (u0 ** 2 + v0 ** 2))) < 1e-10
+    one_minus_pr = 2.5 * case.const["cpr"]
+    qx = np.sum(W * cx * (c2 * G + H))
+    qy = np.sum(W * cy * (c2 * G + H))
+    # the c^6 terms of the Shakhov polynomial integrate to ~1e-8 relative on this
+    # grid; a wrong polynomial coefficient changes q by O(1)
+    assert abs(qx - one_minus_pr * qx0) < 1e-6 * abs(qx0)
+    assert abs(qy - one_minus_pr * qy0) < 1e-6 * abs(qy0)
+
+
+def test_convert_ns_exact_on_linear_fields():
+    g = load_pkg()
+    case = g.cases.cavity_case(10, 8, kn=0.1, dt=1.0)
+    o = load_oracle().OracleSolver(case)
+    o.initialize()
+    m = o.macro()
+    s = m["s"].copy()
+    x1, y1 = case.mesh.x1y1()
+    a, b = 0.3, 0.2
+    X = np.zeros(s.shape[1:])
+    Y = np.zeros(s.shape[1:])
+    X[2:-2, 2:-2] = x1
+    Y[2:-2, 2:-2] = y1
+    T = 1.0 + b * X
+    s[0] = 1.0
+    s[1] = a * Y
+    s[2] = 0.0
+    s[3] = T
+    s[4] = s[5] = s[9] = T
+    s[6] = s[7] = s[8] = 0.0
+    o.upload(s=s)
+    assert o.convert_ns() == 0
+    mm = o.macro()
+    c = case.const
+    omg = float(np.float32(0.81))
+    dudy = a * math.sqrt(2.0 / c["gamma"]) / c["ma"]
+    worst = 0.0
+    for j in range(2, case.ny - 1):
+        for i in range(1, case.nx):
+            mu = T[i + 3, j + 3] ** omg / c["re"]  # S index = cell index + 3
+            t12 = mu * dudy
+            qx = c["cp"] * mu / c["pr"] * b
+            tj = mm["taoj"][:, i + 1, j + 1]
+            qj = mm["qj"][:, i + 1, j + 1]
+            worst = max(worst, abs(tj[2] + t12) / abs(t12), abs(qj[0] + qx) / abs(qx))
+            assert abs(tj[0]) < 1e-12 and abs(tj[1]) < 1e-12 and abs(tj[3]) < 1e-12
+            assert abs(qj[1]) < 1e-12
+    assert worst < 1e-11, worst
