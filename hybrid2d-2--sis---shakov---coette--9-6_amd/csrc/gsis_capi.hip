@@ -171,6 +171,13 @@ int check_err_flag(gsis_handle *h) {
   HIPCHK(hipMemcpyAsync(&e, h->d.err, sizeof(int), hipMemcpyDeviceToHost, h->s));
   HIPCHK(hipStreamSynchronize(h->s));
   if (e & 1) return fail(GSIS_ETEMP, "T < 0.1 in an interior cell (comput_duvtpckw stop)");
+  if (h->p.mode == GSIS_MODE_M1) {
+    int it = 0, bad = 0;
+    int rc = gsis::synthetic_read_iters(h->syn, h->s, &it, &bad);
+    if (rc) return fail(rc, gsis::synthetic_error());
+    h->ns_iter = it;
+    if (bad) return fail(GSIS_ETEMP, "T < 0.1 in an interior cell during the synthetic NS solve");
+  }
   return 0;
 }
 
@@ -493,6 +500,8 @@ int gsis_residual(gsis_handle *h, double res_rms[4]) {
 int gsis_counters(gsis_handle *h, int32_t *kstep, double *tt, int32_t *ns_iter) {
   if (!h) return fail(GSIS_EARG, "NULL handle");
   HIPCHK(hipStreamSynchronize(h->s));
+  int rc = check_err_flag(h);
+  if (rc) return rc;
   if (kstep) *kstep = h->kstep;
   if (tt) *tt = h->tt;
   if (ns_iter) *ns_iter = h->ns_iter;

@@ -20,61 +20,11 @@
  *  - S_GKUA(11) is left 0: the reference accumulates an unassigned FBEI(11)
  *    into it (src/GKUA_DETFP.f90:337-339) and never reads it on the step path.
  */
-#include "gsis_oracle.h"
+#include "gsis_oracle_impl.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
-
-#define PI_D 3.1415926535897932 /* const_var PI, src/COMMONDATA.f90:8 */
-
-/* ---- layout helpers (Fortran column-major with lower bounds) ---- */
-/* FRD(2,-3:nx+3,-3:ny+3,nvi,nvj) */
-#define NI7 (o->nx + 7)
-#define NJ7 (o->ny + 7)
-#define FPLANE ((size_t)2 * NI7 * NJ7)
-#define FV(base, ird, i, j) (base)[((ird)-1) + 2 * (((i) + 3) + (size_t)NI7 * ((j) + 3))]
-/* S_GKUA(11,-3:nx+3,-3:ny+3) */
-#define S_(k, i, j) o->S[((k)-1) + 11 * (((i) + 3) + (size_t)NI7 * ((j) + 3))]
-#define S0_(k, i, j) o->S0[((k)-1) + 11 * (((i) + 3) + (size_t)NI7 * ((j) + 3))]
-#define S00_(k, i, j) o->S00[((k)-1) + 11 * (((i) + 3) + (size_t)NI7 * ((j) + 3))]
-/* (-1:nx+1,-1:ny+1) */
-#define NI3 (o->nx + 3)
-#define C2(a, i, j) (a)[((i) + 1) + (size_t)NI3 * ((j) + 1)]
-/* (4,-1:nx+1,-1:ny+1) */
-#define U4(a, m, i, j) (a)[((m)-1) + 4 * (((i) + 1) + (size_t)NI3 * ((j) + 1))]
-#define Q2(a, m, i, j) (a)[((m)-1) + 2 * (((i) + 1) + (size_t)NI3 * ((j) + 1))]
-/* (nx,ny) 1-based */
-#define G2(a, i, j) (a)[((i)-1) + (size_t)o->nx * ((j)-1)]
-/* node coordinates (-1:nx+2,-1:ny+2) */
-#define NI4 (o->nx + 4)
-#define XN(a, i, j) (a)[((i) + 1) + (size_t)NI4 * ((j) + 1)]
-
-struct orc {
-  int nx, ny, nvi, nvj;
-  double *vi, *vj, *wi, *wj;
-  double *x, *y, *x1, *y1;
-  double *vol, *si, *sj, *ni1, *ni2, *nj1, *nj2;
-  int nbc;
-  gsis_bc *bc;
-  double cfc, x1s, cpr, twall, dt, ma, gamma, re, pr, cv, cp, aoa;
-  int scheme, mode, if_viscous, local_dt, ns_maxit;
-  double cfl, dtmax, dtmin, ns_tol;
-  /* state */
-  double *F, *F0, *F00;
-  double *S, *S0, *S00;
-  double *CF, *U, *U0, *U2;
-  double *taonsi_g, *taonsj_g, *qi_g, *qj_g;
-  double *taonsi_in, *taonsj_in, *qi_in, *qj_in;
-  double *swall; /* per wall/farfield BC: (10, nk) packed in BC order */
-  int *swall_off;
-  /* primitives of comput_duvtpckw (src/sub_NS_singlegrid.f90:121-157) */
-  double *d, *uu, *vv, *T, *p, *amu;
-  /* synthetic NS work */
-  double *res, *du, *dtl, *lci, *lcj, *lvi, *lvj;
-  int kstep, ns_iter;
-  double tt;
-};
 
 static double fminmd(double x, double y) { /* FMINMD, src/Functions.f90:10-26 */
   if ((x > 0. && y > 0.) || (x < 0. && y < 0.)) return (fabs(x) <= fabs(y)) ? x : y;
@@ -714,7 +664,7 @@ void orc_detfp(orc_t *o) {
 }
 
 /* comput_duvtpckw, src/sub_NS_singlegrid.f90:121-157; returns nonzero on T<0.1 */
-static int comput_prims(orc_t *o) {
+int orc_comput_prims(orc_t *o) {
   const double ptmp = 1. / (o->gamma * o->ma * o->ma);
   int bad = 0;
   for (int j = -1; j <= o->ny + 1; j++)
@@ -732,7 +682,7 @@ static int comput_prims(orc_t *o) {
 }
 
 /* get_viscous: mu = T^0.81/Re with the REAL(4) literal 0.81, src/sub_Residual.f90:553-576 */
-static void get_viscous(orc_t *o) {
+void orc_get_viscous(orc_t *o) {
   const double omg = (double)0.81f;
   for (int j = 0; j <= o->ny; j++)
     for (int i = 0; i <= o->nx; i++) C2(o->amu, i, j) = 1. / o->re * pow(C2(o->T, i, j), omg);
@@ -741,7 +691,7 @@ static void get_viscous(orc_t *o) {
 /* Face-centred NS stress and heat flux (the same discretisation is used by
  * Convert_NS src/Hybrid_Sub.f90:257-309,358-407 and by Residual_FVM
  * src/sub_Residual.f90:125-165,262-298).  dir 0: i-face (i,j), 1: j-face. */
-static void ns_face_stress(const orc_t *o, int dir, int i, int j, double *t11, double *t22,
+void orc_ns_face_stress(const orc_t *o, int dir, int i, int j, double *t11, double *t22,
                            double *t12, double *t33, double *qx, double *qy, double *u1o, double *v1o) {
   const int nx = o->nx, ny = o->ny;
   double amu1, amk1, dix, diy, djx, djy, diu, div, dit, dju, djv, djt, u1, v1;
@@ -834,15 +784,15 @@ int orc_convert_ns(orc_t *o) {
     for (int m = 1; m <= 4; m++) U4(o->U, m, i, 0) = 2.0 * U4(o->U, m, i, 1) - U4(o->U, m, i, 2);
   for (int i = -1; i <= nx + 1; i++)
     for (int m = 1; m <= 4; m++) U4(o->U, m, i, ny) = 2.0 * U4(o->U, m, i, ny - 1) - U4(o->U, m, i, ny - 2);
-  int bad = comput_prims(o);
+  int bad = orc_comput_prims(o);
   if (bad) return GSIS_ETEMP;
-  get_viscous(o);
+  orc_get_viscous(o);
   if (!o->if_viscous) return 0;
   const double gm2 = g * (ma * ma);
   for (int j = 1; j <= ny - 1; j++)
     for (int i = 1; i <= nx; i++) {
       double t11, t22, t12, t33, qx, qy;
-      ns_face_stress(o, 0, i, j, &t11, &t22, &t12, &t33, &qx, &qy, NULL, NULL);
+      orc_ns_face_stress(o, 0, i, j, &t11, &t22, &t12, &t33, &qx, &qy, NULL, NULL);
       U4(o->taonsi_in, 1, i, j) = t11; U4(o->taonsi_in, 2, i, j) = t22;
       U4(o->taonsi_in, 3, i, j) = t12; U4(o->taonsi_in, 4, i, j) = t33;
       Q2(o->qi_in, 1, i, j) = qx; Q2(o->qi_in, 2, i, j) = qy;
@@ -868,7 +818,7 @@ int orc_convert_ns(orc_t *o) {
   for (int j = 1; j <= ny; j++)
     for (int i = 1; i <= nx - 1; i++) {
       double t11, t22, t12, t33, qx, qy;
-      ns_face_stress(o, 1, i, j, &t11, &t22, &t12, &t33, &qx, &qy, NULL, NULL);
+      orc_ns_face_stress(o, 1, i, j, &t11, &t22, &t12, &t33, &qx, &qy, NULL, NULL);
       U4(o->taonsj_in, 1, i, j) = t11; U4(o->taonsj_in, 2, i, j) = t22;
       U4(o->taonsj_in, 3, i, j) = t12; U4(o->taonsj_in, 4, i, j) = t33;
       Q2(o->qj_in, 1, i, j) = qx; Q2(o->qj_in, 2, i, j) = qy;

@@ -49,6 +49,7 @@ def load():
         lib.orc_get_frd0.argtypes = [vp, dp]
         lib.orc_get_prims.argtypes = [vp, dp]
         lib.orc_feq.argtypes = [vp]
+        lib.orc_van_leer.argtypes = [dp, dp, ctypes.c_double, dp]
         lib.orc_set_macro.argtypes = [vp, dp, dp, dp, dp]
         lib.orc_get_macro.argtypes = [vp] + [dp] * 9
         lib.orc_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int)]
@@ -66,6 +67,16 @@ def load():
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def van_leer(ql, qr, gamma):
+    """Van Leer flux of the oracle for rotated states (d, un, ut, p)."""
+    lib = load()
+    a = np.ascontiguousarray(ql, dtype=np.float64)
+    b = np.ascontiguousarray(qr, dtype=np.float64)
+    f = np.zeros(4)
+    lib.orc_van_leer(_dp(a), _dp(b), float(gamma), _dp(f))
+    return f
 
 
 class OracleError(RuntimeError):

@@ -78,3 +78,30 @@ def test_iteration_count_to_convergence():
     for (ka, ra), (kb, rb) in zip(hg, hc):
         assert ka == kb and rel_l2(ra, rb) <= 1e-9
     _compare(gpu, cpu, "cavity10 converged", frd=False)
+
+
+@pytest.mark.parametrize("maker", [channel_case, cylinder_case], ids=["channel50x6", "cylinder40"])
+def test_m1_shipped_parity(maker):
+    """Mode M1 (synthetic NS solve on): same fields and inner iteration counts."""
+    case = maker()
+    gpu, cpu = _pair(case, mode=1, ns_maxit=3000)
+    for k in range(2):
+        gpu.step(1)
+        cpu.step(1)
+        kg, _, ig = gpu.counters()
+        kc, _, ic = cpu.counters()
+        print(case.name, "step", k + 1, "inner its gpu", ig, "oracle", ic, flush=True)
+        assert ig == ic
+        _compare(gpu, cpu, f"{case.name} M1 step {k + 1}", frd=False)
+
+
+def test_m1_cavity_parity():
+    g = load_pkg()
+    case = g.cases.cavity_case(16, 24, kn=0.1, lid=0.15, dt=1.0)
+    gpu, cpu = _pair(case, mode=1)
+    for k in range(4):
+        gpu.step(1)
+        cpu.step(1)
+        assert gpu.counters()[2] == cpu.counters()[2]
+    _compare(gpu, cpu, "cavity16 M1 step 4")
+    assert rel_l2(gpu.residual(), cpu.residual()) <= TOL

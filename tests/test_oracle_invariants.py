@@ -223,3 +223,18 @@ def test_convert_ns_exact_on_linear_fields():
             assert abs(tj[0]) < 1e-12 and abs(tj[1]) < 1e-12 and abs(tj[3]) < 1e-12
             assert abs(qj[1]) < 1e-12
     assert worst < 1e-11, worst
+
+
+@pytest.mark.parametrize("mach", [0.3, 0.9, 1.5, -0.6, -2.0])
+def test_van_leer_consistency(mach):
+    """Flux-vector splitting is consistent: F(Q,Q) equals the Euler flux of Q
+    (Flux_Van_Leer_1Da, src/sub_flux_split.f90:141-192)."""
+    orc = load_oracle()
+    g = 5.0 / 3.0
+    d, p, ut = 1.3, 0.7, -0.25
+    a = math.sqrt(g * p / d)
+    un = mach * a
+    q = [d, un, ut, p]
+    f = orc.van_leer(q, q, g)
+    exact = [d * un, d * un * un + p, d * un * ut, un * (g * p / (g - 1.0) + 0.5 * d * (un * un + ut * ut))]
+    assert np.allclose(f, exact, rtol=1e-13, atol=1e-14)
