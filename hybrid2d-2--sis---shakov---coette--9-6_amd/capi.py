@@ -112,6 +112,9 @@ class GsisParams(ctypes.Structure):
         ("dtmin", ctypes.c_double),
         ("ns_tol", ctypes.c_double),
         ("ns_maxit", ctypes.c_int32),
+        ("kstep_show", ctypes.c_int32),
+        ("split_kernels", ctypes.c_int32),
+        ("walk_tiles", ctypes.c_int32),
     ]
 
 

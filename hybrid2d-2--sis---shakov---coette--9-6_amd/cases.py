@@ -452,7 +452,8 @@ class ParamPack:
     """Holds a filled GsisParams and keeps every array it points to alive."""
 
     def __init__(self, case: Case, precision: int = capi.PREC_FP64, mode: int = capi.MODE_M0,
-                 ns_tol: float = float(F32(1e-6)), ns_maxit: int = 0):
+                 ns_tol: float = float(F32(1e-6)), ns_maxit: int = 0, kstep_show: int = 0,
+                 split_kernels: int = 0, walk_tiles: int = 0):
         import ctypes
         m, c = case.mesh, case.const
         self.arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in (case.vi, case.vj, case.wi, case.wj)]
@@ -479,6 +480,9 @@ class ParamPack:
         p.dtmin = float(c.get("dtmin", 1e-7))
         p.ns_tol = float(ns_tol)
         p.ns_maxit = int(ns_maxit)
+        p.kstep_show = int(kstep_show)
+        p.split_kernels = int(split_kernels)
+        p.walk_tiles = int(walk_tiles)
         self.p = p
 
 

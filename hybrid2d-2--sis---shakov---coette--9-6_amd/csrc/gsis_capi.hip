@@ -44,6 +44,7 @@ struct gsis_handle {
   size_t tsize = 8;
   void *FG = nullptr, *FH = nullptr, *slot = nullptr, *RG = nullptr, *RH = nullptr;
   std::vector<void *> allocs;
+  double *cpar = nullptr;  // per-cell constants of the fused kinetic kernel
   int *wall_list = nullptr;
   int nwall = 0;
   double *swall = nullptr;
@@ -93,6 +94,7 @@ int validate(const gsis_params *p) {
     return fail(GSIS_EARG, "precision must be GSIS_PREC_FP64 or GSIS_PREC_FP32");
   if (p->mode != GSIS_MODE_M0 && p->mode != GSIS_MODE_M1) return fail(GSIS_EARG, "unknown mode");
   if (p->dt <= 0.0) return fail(GSIS_EARG, "dt must be positive");
+  if (p->walk_tiles < 0 || p->kstep_show < 0) return fail(GSIS_EARG, "walk_tiles and kstep_show must be >= 0");
   for (int b = 0; b < p->nbc; b++) {
     const gsis_bc &c = p->bc[b];
     bool ok = false;
@@ -142,9 +144,22 @@ void kinetic(gsis_handle *h) {
     if (h->t_pending) { (void)hipEventSynchronize(h->ev[5]); harvest_timing(h); }
     (void)hipEventRecord(h->ev[0], h->s);
   }
-  gsis::launch_rhs<T>(h->d, (const T *)h->FG, (const T *)h->FH, (const T *)h->slot, (T *)h->RG, (T *)h->RH, h->s);
+  const int show = h->p.kstep_show;
+  const int store = (show <= 0 || (h->kstep + 1) % show == 0) ? 1 : 0;
+  T *FG = (T *)h->FG, *FH = (T *)h->FH, *RG = (T *)h->RG, *RH = (T *)h->RH;
+  const T *slot = (const T *)h->slot;
+  if (!h->p.split_kernels) {
+    if (h->timing) { (void)hipEventRecord(h->ev[1], h->s); (void)hipEventRecord(h->ev[2], h->s); }
+    if (gsis::launch_walk<T>(h->d, h->cpar, FG, slot, RG, store, h->p.walk_tiles, h->s)) {
+      if (h->timing) (void)hipEventRecord(h->ev[3], h->s);
+      if (store) gsis::launch_res4<T>(h->d, RG, RH, h->s);
+      return;
+    }
+    if (h->timing) (void)hipEventRecord(h->ev[0], h->s);
+  }
+  gsis::launch_rhs<T>(h->d, FG, FH, slot, RG, RH, h->s);
   if (h->timing) { (void)hipEventRecord(h->ev[1], h->s); (void)hipEventRecord(h->ev[2], h->s); }
-  gsis::launch_sweep<T>(h->d, (T *)h->FG, (T *)h->FH, (const T *)h->RG, (const T *)h->RH, h->s);
+  gsis::launch_sweep<T>(h->d, FG, FH, RG, RH, h->s);
   if (h->timing) (void)hipEventRecord(h->ev[3], h->s);
 }
 
@@ -268,6 +283,7 @@ int gsis_create(const gsis_params *p, gsis_handle **out) {
       (rc = dalloc(h, &d.taoi, 8 * 4 * N)) || (rc = dalloc(h, &d.taoj, 8 * 4 * N)) ||
       (rc = dalloc(h, &d.qi, 8 * 2 * N)) || (rc = dalloc(h, &d.qj, 8 * 2 * N)) ||
       (rc = dalloc(h, &d.prim, 8 * 5 * N)) || (rc = dalloc(h, &d.res4, 8 * 4 * N)) || (rc = dalloc(h, &d.err, 64)) ||
+      (rc = dalloc(h, &h->cpar, 8 * gsis::kNCellPar * N)) ||
       (rc = dalloc(h, &h->resbuf, 8 * 8)))
     return cleanup(rc);
   {  // Creat_Mesh sets U, U0, U2 to (1,0,0,1) (src/sub_init.f90:159-162)
@@ -324,9 +340,12 @@ int gsis_create(const gsis_params *p, gsis_handle **out) {
   // distributions
   const size_t nf = (size_t)d.ncell * d.nv * h->tsize;
   const size_t ns = (size_t)4 * d.L * 2 * d.nv * h->tsize;
-  if ((rc = dalloc(h, &h->FG, nf)) || (rc = dalloc(h, &h->FH, nf)) || (rc = dalloc(h, &h->slot, ns)) ||
-      (rc = dalloc(h, &h->RG, nf)) || (rc = dalloc(h, &h->RH, nf)))
-    return cleanup(rc);
+  // G plane, H plane and the wall slots in one allocation (the fused kernel
+  // addresses all three with 32-bit offsets from one base)
+  if ((rc = dalloc(h, &h->FG, 2 * nf + ns)) || (rc = dalloc(h, &h->RG, 2 * nf))) return cleanup(rc);
+  h->FH = (char *)h->FG + nf;
+  h->slot = (char *)h->FG + 2 * nf;
+  h->RH = (char *)h->RG + nf;
   if (p->mode == GSIS_MODE_M1) {
     rc = gsis::synthetic_init(h->d, *p, h->syn);
     if (rc) return cleanup(fail(rc, gsis::synthetic_error()));

@@ -12,6 +12,11 @@ template <typename T> void launch_periodic(const Dev &d, const BcDev &b, T *FG, 
 template <typename T> void launch_rhs(const Dev &d, const T *FG, const T *FH, const T *slot, T *RG, T *RH, hipStream_t s);
 template <typename T> void launch_sweep(const Dev &d, T *FG, T *FH, const T *RG, const T *RH, hipStream_t s);
 template <typename T> void launch_moments(const Dev &d, T *FG, T *FH, hipStream_t s);
+constexpr int kNCellPar = 20;  // per-cell constants of the fused kinetic kernel
+// Fused kinetic step over F = [G plane | H plane] (and R likewise); false when
+// the mesh does not fit the fused kernel.
+template <typename T> bool launch_walk(const Dev &d, double *cpar, T *F, const T *slot, T *R, int store_rhs, int tiles, hipStream_t s);
+template <typename T> void launch_res4(const Dev &d, const T *RG, const T *RH, hipStream_t s);
 template <typename T> void launch_frd_in(const Dev &d, const double *frd, T *FG, T *FH, T *slot, hipStream_t s);
 template <typename T> void launch_frd_out(const Dev &d, double *frd, const T *FG, const T *FH, const T *slot, hipStream_t s);
 void launch_save_s0(const Dev &d, hipStream_t s);

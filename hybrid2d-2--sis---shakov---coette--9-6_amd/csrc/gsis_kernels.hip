@@ -4,6 +4,8 @@
 // layout is described in gsis_device.h.  Storage/compute type T is double for
 // the FP64 path and float for the FP32-sweep path (moments, wall sums and the
 // macroscopic fields stay FP64 in both).
+#include <type_traits>
+
 #include "gsis_kernels.h"
 #include "gsis_ns_common.h"
 
@@ -14,8 +16,9 @@ namespace gsis {
 // ------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T fminmd(T x, T y) {  // FMINMD, src/Functions.f90:10-26
-  if ((x > T(0) && y > T(0)) || (x < T(0) && y < T(0))) return (fabs(x) <= fabs(y)) ? x : y;
-  return T(0);
+  const bool same = (x > T(0) & y > T(0)) | (x < T(0) & y < T(0));
+  const T m = (fabs(x) <= fabs(y)) ? x : y;
+  return same ? m : T(0);
 }
 template <typename T>
 __device__ __forceinline__ T posp(T a) { return (a + fabs(a)) * T(0.5); }
@@ -456,6 +459,384 @@ __global__ void __launch_bounds__(256) k_sweep(Dev d, int VT, T *FG, T *FH, cons
 }
 
 // ------------------------------------------------------------------------
+// Per-cell constants of the kinetic step, computed once per step so that the
+// per-velocity work needs no division, sorted by "wrapped anti-diagonal":
+// cell (i,j) goes to plane r = (i+j-2) mod (ny-1), slot i-1, as
+// cpar[r][i-1][k].  Each plane then
+// holds exactly one cell per mesh column: the cells the fused kernel visits
+// in one step.  Values (k): 0 n/(pi T), 1 cpr/(n T^2) (Get_Feq,
+// src/GKUA_Residual.f90:449-450), 2 1/T, 3 T, 4 u, 5 v, 6 qx, 7 qy, 8 CF,
+// 9 1/vol, 10 vol, 11 vol/dt + vol*CF (the velocity-independent part of the
+// LU-SGS diagonal, src/GKUA_Residual.f90:43), then s*(n1,n2) of the faces:
+// 12-13 i-face i, 14-15 i-face i+1, 16-17 j-face j, 18-19 j-face j+1.
+// ------------------------------------------------------------------------
+constexpr int kNW = kNCellPar;
+__global__ void k_cellpar(Dev d, double *cpar) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncol = d.nx - 1, nrow = d.ny - 1;
+  if (t >= ncol * nrow) return;
+  const int i = t % ncol + 1, j = t / ncol + 1;
+  const int c = cidx(d, i, j), N = d.ncell;
+  const int r = (i + j - 2) % nrow;
+  double *o = cpar + ((size_t)r * ncol + (i - 1)) * kNW;
+  const double n = d.S[c], T = d.S[3 * N + c];
+  const int g0 = gidx(d, i, j), gi = gidx(d, i + 1, j), gj = gidx(d, i, j + 1);
+  const double vol = d.vol[g0], cf = d.CF[c];
+  const double vals[kNW] = {n / (kPi * T), d.cpr / (n * T * T), 1.0 / T, T, d.S[N + c], d.S[2 * N + c],
+                            d.S[7 * N + c], d.S[8 * N + c], cf, 1.0 / vol, vol, vol / d.dt + vol * cf,
+                            d.ni1[g0] * d.si[g0], d.ni2[g0] * d.si[g0], d.ni1[gi] * d.si[gi], d.ni2[gi] * d.si[gi],
+                            d.nj1[g0] * d.sj[g0], d.nj2[g0] * d.sj[g0], d.nj1[gj] * d.sj[gj], d.nj2[gj] * d.sj[gj]};
+#pragma unroll
+  for (int k = 0; k < kNW; k++) o[k] = vals[k];
+}
+
+// Flux through one face in the fused kernel: the rules of iface_flux /
+// jface_flux written without branches.  fl holds the override flags of the
+// row (i-faces) or column (j-faces): bit 0 low boundary face, bit 1 high one;
+// sv is the wall-slot value of the boundary face.  Next to an overridden face
+// the limiter on that side is dropped (flux_lowadj / flux_highadj), on it the
+// flux is a * slot (src/GKUA_Residual.f90:200-262).
+template <typename T>
+__device__ __forceinline__ T wflux(int f, int n, int fl, T a, T fm2, T fm1, T f0, T fp1, T sv, bool nnd) {
+  const T asp = posp(a), asn = negp(a);
+  T ul = fm1, ur = f0;
+  if (nnd) {
+    const T d0 = f0 - fm1;
+    const T liml = ((fl & 1) && f == 2) ? d0 : fminmd(d0, fm1 - fm2);
+    const T limr = ((fl & 2) && f == n - 1) ? d0 : fminmd(d0, fp1 - f0);
+    ul = fm1 + T(0.5) * liml;
+    ur = f0 - T(0.5) * limr;
+  }
+  const T fx = asp * ul + asn * ur;
+  const bool wall = ((fl & 1) && f == 1) || ((fl & 2) && f == n);
+  return wall ? a * sv : fx;
+}
+// exp(x) for the equilibrium, x <= 0.  Cody-Waite reduction x = k ln2 + r
+// (|r| <= ln2/2, ln2 split into the fdlibm high/low parts so that k*ln2_hi is
+// exact) and the degree-13 Taylor polynomial of e^r in Horner form
+// (truncation error < 5e-18 relative); exp(x) = 2^k p(r).  Written out here
+// rather than calling the library exp because its coefficients stay in
+// scalar registers, loaded per call from constant memory through a pointer
+// the compiler cannot hoist: the library routine's constants, hoisted out of
+// the fused kernel's loop, cost ~40 vector registers and forced spills.
+// Accuracy is checked end to end: the fused kernel's fields agree with the
+// oracle (libm exp) to <= 1e-10 relative L2.
+__constant__ double kExpC[16] = {1.0 / 6227020800.0, 1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0,
+                                 1.0 / 362880.0,     1.0 / 40320.0,     1.0 / 5040.0,     1.0 / 720.0,
+                                 1.0 / 120.0,        1.0 / 24.0,        1.0 / 6.0,        0.5,
+                                 1.4426950408889634, -6.93147180369123816490e-01, -1.90821492927058770002e-10,
+                                 -746.0};
+typedef const __attribute__((address_space(4))) double *cdptr;
+__device__ __forceinline__ double wexp(double x) {
+  cdptr kc = (cdptr)kExpC;
+  asm volatile("" : "+s"(kc));
+  x = x < kc[15] ? kc[15] : x;  // below -746 the result underflows to 0 anyway
+  const double k = rint(x * kc[12]);
+  double r = fma(k, kc[13], x);
+  r = fma(k, kc[14], r);
+  double p = kc[0];
+#pragma unroll
+  for (int m = 1; m < 12; m++) p = fma(p, r, kc[m]);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)k);
+}
+__device__ __forceinline__ float wexp(float x) { return expf(x); }
+// element at a 32-bit byte offset from a uniform base (global_load with an
+// SGPR base and a VGPR offset)
+template <typename T>
+__device__ __forceinline__ T ldo(const T *base, unsigned off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T *base, unsigned off, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
+// ------------------------------------------------------------------------
+// Fused kinetic step ("column walker"): Shakhov equilibrium, NND fluxes,
+// source, forward LU-SGS recursion, non-recursive backward pass, update and
+// interior clamp in ONE pass over F (src/GKUA_Residual.f90:26-109,117-469).
+//
+// G and H are independent transport problems in this step, so a tile is one
+// of them for VT consecutive velocities; F holds the G plane followed by the
+// H plane.  Thread (column i, lane) of a block owns mesh column i and one
+// velocity of the tile.  A block walks K tiles back to back: at step t column
+// i is at position q = t - (i-1) of its sequence (tile q / (ny-1), row
+// q mod (ny-1) + 1), so all active columns sit on the same wrapped
+// anti-diagonal and the forward recursion's inputs from column i-1 (its
+// face flux and Delta*) were produced one step earlier and come through LDS,
+// while the j-stencil F(i, j-2..j+2) slides in registers.  Column i finalises
+// position q-2 at step t (backward pass + update): it needs Delta*(i+1, .),
+// published by column i+1 at step t-1, and its own Delta* of the next row,
+// kept with the backward coefficients in a 3-deep LDS ring.  F is updated in
+// place: every other reader of a value has consumed it at an earlier step
+// (column i+1 reads it at t+1 and waits for it before the barrier that ends
+// t+1; the write is at t+2).  All global loads of a position are issued one
+// step ahead, and the velocity-independent constants of the next step's
+// cells are staged cooperatively into LDS one step ahead.
+// Offsets are 32-bit byte offsets from F, which holds the G plane, the H
+// plane and the wall slots (slot at byte offset slotb) in one allocation of
+// less than 4 GiB.
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT, int K, T *F, unsigned planeb,
+                                               unsigned slotb, T *R, int store_rhs) {
+  using T2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+  constexpr unsigned ES = sizeof(T);
+  extern __shared__ double lds_d[];
+  const int ncol = d.nx - 1, nrow = d.ny - 1, nv = d.nv, nx = d.nx, ny = d.ny;
+  const int nthr = blockDim.x;
+  const int nconst = kNW * ncol;
+  double *C = lds_d;                                            // [2][ncol][kNW]
+  double2 *V = reinterpret_cast<double2 *>(C + 2 * nconst);     // [K][VT] velocities of the tiles
+  T2 *X = reinterpret_cast<T2 *>(V + K * VT);                   // [2][ncol][VT] (face flux, Delta*)
+  T *H = reinterpret_cast<T *>(X + 2 * ncol * VT);              // [3][3][nthr] history ring
+  uint8_t *OF = reinterpret_cast<uint8_t *>(H + 9 * nthr);      // override flags, rows and columns
+  const int ntiles = 2 * ((nv + VT - 1) / VT);
+  const int tile0 = blockIdx.x * K;
+  const int ktiles = min(K, ntiles - tile0);
+  const bool nnd = d.scheme != 0;
+  const unsigned cb = (unsigned)nv * ES;       // one cell
+  const unsigned rb = (unsigned)d.NI * cb;     // one row of cells
+  const unsigned L2 = 2u * d.L;
+  const int vshift = 31 - __builtin_clz(VT);   // VT is a power of two
+  {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < ktiles * VT; e += nthr) {
+      const int tau = tile0 + e / VT, v = (tau >> 1) * VT + e % VT;
+      V[e] = v < nv ? make_double2(d.vi[v % d.nvi], d.vj[v / d.nvi]) : make_double2(0.0, 0.0);
+    }
+    // bits 0-1: i-face overrides of row e; bits 2-3: j-face overrides of column e
+    const int nof = max(nrow + 2, d.nx + 2);
+    for (int e = tid; e < nof; e += nthr)
+      OF[e] = (e <= nrow + 1 ? ((d.ovrI1[e] ? 1 : 0) | (d.ovrI3[e] ? 2 : 0)) : 0) |
+              (e < d.nx + 2 ? ((d.ovrJ2[e] ? 4 : 0) | (d.ovrJ4[e] ? 8 : 0)) : 0);
+    for (int e = tid; e < 2 * nconst; e += nthr) C[e] = cpar[nrow > 1 || e < nconst ? e : e - nconst];
+  }
+  // Per-thread indices are recomputed inside the loop from a laundered copy
+  // of threadIdx.x: keeping them live across the loop costs more registers
+  // than the few integer instructions that recompute them.
+  auto tile_of = [&](int kt, int lane, unsigned &tb, int &ird, int &vv) {
+    const int tau = tile0 + kt, v = (tau >> 1) * VT + lane;
+    ird = tau & 1;
+    const bool act = v < nv;
+    vv = act ? v : 0;
+    tb = (unsigned)ird * planeb + (unsigned)vv * ES;
+    return act;
+  };
+  // byte offset (from F) of slot[((face-1)*L + pos)*2 + ird][v]
+  auto soff = [&](unsigned face, unsigned pos, unsigned ird, unsigned v) {
+    return slotb + (((face - 1) * L2 + 2 * pos + ird) * (unsigned)nv + v) * ES;
+  };
+  // values of the next position, loaded one step ahead; n0..n3 hold
+  // F(i,-1..2) of the next tile, which arrive one row per step (via pn, for
+  // slot pm) during the last four rows of a tile
+  T n0 = T(0), n1 = T(0), n2 = T(0), n3 = T(0), pw = T(0), pm1 = T(0), pp1 = T(0), pp2 = T(0), px = T(0), py = T(0);
+  T pn = T(0);
+  int pm = -1;
+  // loads of position (kn, jn) of column i: the stencil values beside the
+  // window and the wall-slot / ghost values of the boundary faces (addresses
+  // that a column does not need point at its own cell); returns the offset
+  // of cell (i, jn)
+  auto load_pos = [&](int i, int lane, int kn, int jn) {
+    unsigned tb;
+    int ird, vv;
+    tile_of(kn, lane, tb, ird, vv);
+    const unsigned oc = tb + (unsigned)(jn + 1) * rb + (unsigned)(i + 1) * cb;
+    const int rf = OF[jn], cf = OF[i] >> 2;
+    const unsigned ax = i == 1 ? ((rf & 1) ? soff(1, jn + 1, ird, vv) : oc - 2 * cb)
+                               : ((i == ncol && (rf & 2)) ? soff(3, jn + 1, ird, vv) : oc);
+    const unsigned ay = (jn == 1 && (cf & 1)) ? soff(2, i + 1, ird, vv)
+                                              : ((jn == nrow && (cf & 2)) ? soff(4, i + 1, ird, vv) : oc);
+    pw = ldo(F, oc + 2 * rb);
+    pm1 = ldo(F, oc - cb);
+    pp1 = ldo(F, oc + cb);
+    pp2 = ldo(F, oc + 2 * cb);
+    px = ldo(F, ax);
+    py = ldo(F, ay);
+    return oc;
+  };
+  __syncthreads();  // OF, V and the first two constants planes
+  {
+    const int tl = threadIdx.x, lane = tl & (VT - 1), i = (tl >> vshift) + 1;
+    const unsigned oc = load_pos(i, lane, 0, 1);
+    n0 = ldo(F, oc - 2 * rb);
+    n1 = ldo(F, oc - rb);
+    n2 = ldo(F, oc);
+    n3 = ldo(F, oc + rb);
+  }
+
+  T w0 = T(0), w1 = T(0), w2 = T(0), w3 = T(0), w4 = T(0);  // F(i, j-2..j+2)
+  T wp0 = T(0), wp1 = T(0);  // F(i, ny-2), F(i, ny-1) of the previous tile
+  T fly_prev = T(0);
+  int jc = 0, kc = 0;  // row and tile of the current position
+  const int nseq = ktiles * nrow;
+  const int tsteps = (ncol - 1) + nseq + 2;
+  // The constants of step s are loaded at step s-2 and written to LDS buffer
+  // s&1 at the top of step s-1, after that step's wait for the loads issued
+  // one step earlier; planes 0 and 1 are in place before the loop.
+  int rstage = 2 % nrow;
+  double stg[3] = {0.0, 0.0, 0.0};
+  for (int t = 0; t < tsteps; t++) {
+    int tl = threadIdx.x;
+    asm volatile("" : "+v"(tl));
+    const int lane = tl & (VT - 1), col = tl >> vshift, i = col + 1;
+    const int buf = t & 1, pbuf = buf ^ 1;
+    const int q = t - col;
+    if (t > 0) {
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        if (tl + k * nthr < nconst) C[pbuf * nconst + tl + k * nthr] = stg[k];
+    }
+    {
+      const double *src = cpar + (size_t)rstage * nconst;
+#pragma unroll
+      for (int k = 0; k < 3; k++) stg[k] = src[min(tl + k * nthr, nconst - 1)];
+    }
+    rstage = rstage + 1 == nrow ? 0 : rstage + 1;
+    // the next tile's window: slot pm receives the value loaded one step ago
+    n0 = pm == 0 ? pn : n0;
+    n1 = pm == 1 ? pn : n1;
+    n2 = pm == 2 ? pn : n2;
+    n3 = pm == 3 ? pn : n3;
+    const bool active = q >= 0 && q < nseq;
+    if (active) {
+      if (q == 0) { jc = 1; kc = 0; }
+      else if (jc == nrow) { jc = 1; kc++; }
+      else jc++;
+    }
+    const int j = jc;
+    const bool first = j == 1;
+    // this position's inputs (loaded one step ago), then the next position's loads
+    if (active && first) { wp0 = w1; wp1 = w2; }
+    const T fm1 = pm1, fp1 = pp1, fp2 = pp2, xs = px, ys = py;
+    if (active) {
+      w0 = first ? n0 : w1;
+      w1 = first ? n1 : w2;
+      w2 = first ? n2 : w3;
+      w3 = first ? n3 : w4;
+      w4 = pw;
+    }
+    // Loads of the next position, issued unconditionally (addresses that are
+    // not needed point at the position's own cell) so that the only wait for
+    // them is at the top of the next step.  Before a column starts, the next
+    // position is its first one; after its last, the loads are not used.
+    {
+      const bool started = q >= 0;
+      const bool nvalid = active && q + 1 < nseq;
+      const bool wrap = j == nrow;
+      const int kn = !started ? 0 : (nvalid && wrap ? kc + 1 : kc);
+      const int jn = !started ? 1 : (nvalid ? (wrap ? 1 : j + 1) : max(j, 1));
+      const unsigned oc = load_pos(i, lane, kn, jn);
+      // rows nrow-3..nrow of a tile fetch rows -1..2 of the next tile
+      const int m = j - (nrow - 3);
+      const bool wantn = nvalid && m >= 0 && kc + 1 < ktiles;
+      unsigned tb1;
+      int ird1, vv1;
+      tile_of(kc + 1, lane, tb1, ird1, vv1);
+      pm = wantn ? m : -1;
+      pn = ldo(F, wantn ? tb1 + (unsigned)m * rb + (unsigned)(i + 1) * cb : oc);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (active) {
+      unsigned tb;
+      int ird, vv;
+      const bool act = tile_of(kc, lane, tb, ird, vv);
+      const double2 vxy = V[kc * VT + lane];
+      const T vx = T(vxy.x), vy = T(vxy.y);
+      const double *cc = C + (buf * ncol + col) * kNW;
+      // Shakhov equilibrium of cell (i,j) (Get_Feq, src/GKUA_Residual.f90:420-469)
+      T feq;
+      {
+        const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
+        const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
+        const T fvx = T(cc[1]) * (viu * T(cc[6]) + vjv * T(cc[7]));
+        const T gd = T(cc[0]) * wexp(-vij2);
+        feq = ird == 0 ? gd * (T(1) + fvx * (T(2) * vij2 - T(4)))
+                       : (T(cc[3]) * gd / T(2)) * (T(1) + fvx * (T(2) * vij2 - T(2)));
+      }
+      const int rf = OF[j], cf = OF[i] >> 2;
+      const T axl = vx * T(cc[12]) + vy * T(cc[13]);
+      const T axr = vx * T(cc[14]) + vy * T(cc[15]);
+      T flxl, dl = T(0);
+      if (i == 1) {
+        flxl = wflux(1, nx, rf, axl, xs, fm1, w2, fp1, xs, nnd);
+      } else {
+        const T2 xl = X[(pbuf * ncol + col - 1) * VT + lane];
+        flxl = xl.x;
+        dl = xl.y;
+      }
+      const T flxr = wflux(i + 1, nx, rf, axr, fm1, w2, fp1, fp2, xs, nnd);
+      const T ayb = vx * T(cc[16]) + vy * T(cc[17]);
+      const T ayt = vx * T(cc[18]) + vy * T(cc[19]);
+      const T flyb = first ? wflux(1, ny, cf, ayb, w0, w1, w2, w3, ys, nnd) : fly_prev;
+      const T flyt = wflux(j + 1, ny, cf, ayt, w1, w2, w3, w4, ys, nnd);
+      fly_prev = flyt;
+      const T r = -(((flxr - flxl) + flyt) - flyb) * T(cc[9]) + T(cc[8]) * (feq - w2);
+      if (store_rhs && act) sto(R, tb + (unsigned)(j + 1) * rb + (unsigned)(i + 1) * cb, r);
+      // forward recursion (src/GKUA_Residual.f90:39-57); Delta*(i,0) = Delta*(0,j) = 0
+      const T alfa = T(cc[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
+      const T ra = T(1) / alfa;
+      const int hs = t % 3;  // history ring slot of position q
+      const T db = first ? T(0) : H[(((t + 2) % 3) * 3) * nthr + tl];
+      const T ds = (T(cc[10]) * r + posp(axl) * dl + posp(ayb) * db) * ra;
+      T2 xo;
+      xo.x = flxr;
+      xo.y = ds;
+      X[(buf * ncol + col) * VT + lane] = xo;
+      H[(hs * 3 + 0) * nthr + tl] = ds;
+      H[(hs * 3 + 1) * nthr + tl] = -negp(axr) * ra;
+      H[(hs * 3 + 2) * nthr + tl] = -negp(ayt) * ra;
+    }
+    // backward pass + update of position q-2 (src/GKUA_Residual.f90:64-109)
+    if (q >= 2 && q < nseq + 2) {
+      const bool live = q < nseq;
+      const int jf = live ? (jc > 2 ? jc - 2 : jc - 2 + nrow) : nrow - 1 + (q - nseq);
+      const bool prev = live && jc <= 2;
+      const int h2 = (t + 1) % 3, h1 = (t + 2) % 3;  // slots of positions q-2 and q-1
+      const T dr = (i < ncol) ? X[(pbuf * ncol + col + 1) * VT + lane].y : T(0);
+      // Delta*(i, jf+1) is the value of position q-1, 0 past the last row
+      const T dn = (jf == nrow) ? T(0) : H[(h1 * 3) * nthr + tl];
+      const T du = H[(h2 * 3) * nthr + tl] + (H[(h2 * 3 + 1) * nthr + tl] * dr + H[(h2 * 3 + 2) * nthr + tl] * dn);
+      const T fold = prev ? (jc == 1 ? wp0 : wp1) : (live ? w0 : (q == nseq ? w1 : w2));
+      unsigned tb;
+      int ird, vv;
+      if (tile_of(prev ? kc - 1 : kc, lane, tb, ird, vv)) {
+        T f = fold + du;
+        if ((double)f < kClamp) f = T(kClamp);
+        sto(F, tb + (unsigned)(jf + 1) * rb + (unsigned)(i + 1) * cb, f);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Conserved moments of the stored RHS per interior cell (GaussLegendre_rs,
+// src/GKUA_DETFP_RES.f90:81-120); one block per interior cell.
+template <typename T>
+__global__ void __launch_bounds__(256) k_res4(Dev d, const T *RG, const T *RH) {
+  const int t = blockIdx.x;
+  const int i = t % (d.nx - 1) + 1, j = t / (d.nx - 1) + 1;
+  const int c = cidx(d, i, j);
+  const size_t o = (size_t)c * d.nv;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
+    const int iv = v % d.nvi, jv = v / d.nvi;
+    const double vxd = d.vi[iv], vyd = d.vj[jv], w = d.wi[iv] * d.wj[jv];
+    const double g = (double)RG[o + v], h = (double)RH[o + v];
+    const double vx1 = vxd * g, vy1 = vyd * g;
+    acc[0] += w * g;
+    acc[1] += w * vx1;
+    acc[2] += w * vy1;
+    acc[3] += w * (h + vxd * vx1 + vyd * vy1);
+  }
+  block_sum<4>(acc);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; k++) d.res4[k * d.ncell + c] = acc[k];
+}
+
+// ------------------------------------------------------------------------
 // Positivity clamp of the ghost ring (src/GKUA_Residual.f90:101-109) and the
 // velocity moments with the DETFP conversion and collision frequency
 // (GaussLegendre src/GKUA_DETFP.f90:288-363, DETFP :52-105).  The reference
@@ -799,6 +1180,69 @@ void launch_sweep(const Dev &d, T *FG, T *FH, const T *RG, const T *RH, hipStrea
   hipLaunchKernelGGL(k_sweep<T>, dim3(nb), dim3(256), lds, s, d, vt, FG, FH, RG, RH);
 }
 template <typename T>
+void launch_res4(const Dev &d, const T *RG, const T *RH, hipStream_t s) {
+  hipLaunchKernelGGL(k_res4<T>, dim3((d.nx - 1) * (d.ny - 1)), dim3(256), 0, s, d, RG, RH);
+}
+// Launch shape of the fused kernel: VT velocities per column (a power of two,
+// at least 8 so that the staged constants are at most 3 per thread), K tiles
+// per block so that the grid covers the CUs about once.  False when the mesh
+// or the dynamic LDS does not fit (the caller then uses the split kernels).
+template <typename T>
+bool walk_config(const Dev &d, int tiles, int *vt_out, int *k_out, size_t *lds_out) {
+  static int ncu = 0, max_lds = 0;
+  if (!ncu) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+      ncu = 0;
+      return false;
+    }
+  }
+  const int ncol = d.nx - 1, nrow = d.ny - 1;
+  if (ncol * 8 > 1024) return false;
+  // 32-bit byte offsets into F (two planes) and the wall slots
+  if (2.0 * d.ncell * d.nv * sizeof(T) >= 4294967295.0 || 8.0 * d.L * d.nv * sizeof(T) >= 4294967295.0) return false;
+  int vt = 64;
+  while (vt > 8 && vt * ncol > 1024) vt >>= 1;
+  const int ntiles = 2 * ((d.nv + vt - 1) / vt);
+  const int K = tiles > 0 ? (tiles < ntiles ? tiles : ntiles) : (ntiles / ncu > 1 ? ntiles / ncu : 1);
+  const size_t nthr = (size_t)vt * ncol;
+  const size_t lds = 8 * (2 * (size_t)kNW * ncol + 2 * (size_t)K * vt) + 2 * (size_t)ncol * vt * 2 * sizeof(T) +
+                     9 * nthr * sizeof(T) + (size_t)std::max(nrow + 2, d.nx + 2);
+  if (lds > (size_t)max_lds) return false;
+  static bool attr_set = false;
+  if (lds > 65536 && !attr_set) {
+    if (hipFuncSetAttribute((const void *)k_walk<T>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds) !=
+        hipSuccess)
+      return false;
+    attr_set = true;
+  }
+  *vt_out = vt;
+  *k_out = K;
+  *lds_out = lds;
+  return true;
+}
+template <typename T>
+bool launch_walk(const Dev &d, double *cpar, T *F, const T *slot, T *R, int store_rhs, int tiles, hipStream_t s) {
+  int vt = 0, K = 0;
+  size_t lds = 0;
+  if (!walk_config<T>(d, tiles, &vt, &K, &lds)) return false;
+  // the wall slots must follow the two F planes in the same allocation
+  const ptrdiff_t so = reinterpret_cast<const char *>(slot) - reinterpret_cast<const char *>(F);
+  const size_t slot_bytes = (size_t)8 * d.L * d.nv * sizeof(T);
+  if (so < (ptrdiff_t)(2 * (size_t)d.ncell * d.nv * sizeof(T)) || (size_t)so + slot_bytes > 4294967295u) return false;
+  const int ncl = (d.nx - 1) * (d.ny - 1);
+  hipLaunchKernelGGL(k_cellpar, dim3((ncl + 255) / 256), dim3(256), 0, s, d, cpar);
+  const int ntiles = 2 * ((d.nv + vt - 1) / vt);
+  const int nb = (ntiles + K - 1) / K;
+  const unsigned planeb = (unsigned)((size_t)d.ncell * d.nv * sizeof(T));
+  hipLaunchKernelGGL(k_walk<T>, dim3(nb), dim3(vt * (d.nx - 1)), lds, s, d, cpar, vt, K, F, planeb, (unsigned)so, R,
+                     store_rhs);
+  return true;
+}
+
+template <typename T>
 void launch_moments(const Dev &d, T *FG, T *FH, hipStream_t s) {
   hipLaunchKernelGGL(k_moments<T>, dim3(d.ncell), dim3(256), 0, s, d, FG, FH);
 }
@@ -842,6 +1286,8 @@ void launch_residual(const Dev &d, double *out, hipStream_t s) {
   template void launch_rhs<T>(const Dev &, const T *, const T *, const T *, T *, T *, hipStream_t); \
   template void launch_sweep<T>(const Dev &, T *, T *, const T *, const T *, hipStream_t);        \
   template void launch_moments<T>(const Dev &, T *, T *, hipStream_t);                            \
+  template bool launch_walk<T>(const Dev &, double *, T *, const T *, T *, int, int, hipStream_t);      \
+  template void launch_res4<T>(const Dev &, const T *, const T *, hipStream_t);                  \
   template void launch_frd_in<T>(const Dev &, const double *, T *, T *, T *, hipStream_t);        \
   template void launch_frd_out<T>(const Dev &, double *, const T *, const T *, const T *, hipStream_t);
 INST(double)

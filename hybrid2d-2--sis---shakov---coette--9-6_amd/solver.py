@@ -23,6 +23,7 @@ class GsisSolver:
         self.lib = capi.load_library()
         self.pack = ParamPack(case, precision=precision, mode=mode, **kw)
         self.mode = mode
+        self.kstep_show = int(kw.get("kstep_show", 0))
         h = ctypes.c_void_p()
         capi.check(self.lib, self.lib.gsis_create(ctypes.byref(self.pack.p), ctypes.byref(h)))
         self.h = h
@@ -93,6 +94,8 @@ class GsisSolver:
         """Step until all four RMS residuals are below ``tol`` (checked every
         ``show`` steps, as DETFP_res is called every Kstep_show steps).
         Returns (steps, history[(kstep, res[4])])."""
+        if self.kstep_show and show % self.kstep_show:
+            raise ValueError("show must be a multiple of the handle's kstep_show")
         hist = []
         done = 0
         while done < max_steps:

@@ -93,6 +93,12 @@ typedef struct gsis_params {
   double cfl, dtmax, dtmin;
   double ns_tol;    /* inner loop runs while max |Res_4| > ns_tol (reference 1e-6) */
   int32_t ns_maxit; /* safety cap on inner iterations (0 = 100000) */
+  /* residual cadence (Kstep_show, src/opencfd_ec2d.f90:47): the kinetic RHS
+     of a step is kept for gsis_residual only when the step count after it is a
+     multiple of kstep_show; 0 keeps it after every step */
+  int32_t kstep_show;
+  int32_t split_kernels; /* 1: separate RHS / sweep kernels instead of the fused one */
+  int32_t walk_tiles;    /* velocity tiles per workgroup of the fused kernel; 0 = automatic */
 } gsis_params;
 
 typedef struct gsis_handle gsis_handle;
@@ -121,8 +127,9 @@ int gsis_refresh_moments(gsis_handle *h);
  * Asynchronous: returns once the work is queued on the handle's stream. */
 int gsis_step(gsis_handle *h, int32_t nsteps);
 
-/* RMS of the four conserved moments of the last step's kinetic RHS,
- * B%Res_rms(1:4) of DETFP_res (src/GKUA_DETFP_RES.f90:45-54).
+/* RMS of the four conserved moments of the kinetic RHS of the last step
+ * whose RHS was kept (see kstep_show), B%Res_rms(1:4) of DETFP_res
+ * (src/GKUA_DETFP_RES.f90:45-54).
  * Returns GSIS_ENAN when res_rms[0] is NaN (src/IO.f90:236-244). */
 int gsis_residual(gsis_handle *h, double res_rms[4]);
 
