@@ -56,6 +56,20 @@ def cpu_baseline(case, steps):
     return case.ncell * case.nv * steps / (t1 - t0), t1 - t0
 
 
+def copy_probe(nbytes):
+    """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE of
+    its dispatches give the factor between the counters and HBM bytes."""
+    import torch
+
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,12 +85,17 @@ def main():
     ap.add_argument("--kstep-show", type=int, default=50,
                     help="residual cadence (the reference's Kstep_show; the kinetic RHS is kept every N steps)")
     ap.add_argument("--split", action="store_true", help="separate RHS and sweep kernels instead of the fused one")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only copy a device buffer of BYTES bytes 5 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the velocity-space sharding across GPUs is described in DESIGN.md, not built.", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     pkg = _load("gsis_amd", PKG_DIR / "__init__.py", PKG_DIR)
     solver = _load("gsis_amd.solver", PKG_DIR / "solver.py")
     case = make_case(pkg, args.n, args.nv, args.kn, args.lid, args.dt)

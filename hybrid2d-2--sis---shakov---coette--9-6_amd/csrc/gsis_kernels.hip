@@ -14,16 +14,19 @@ namespace gsis {
 // ------------------------------------------------------------------------
 // small helpers
 // ------------------------------------------------------------------------
+// FMINMD (src/Functions.f90:10-26): the argument of smaller magnitude when
+// both have the same sign, else 0.  Written as (sign(x)+sign(y))/2 * min(|x|,|y|),
+// which gives the same values without branches (a zero argument gives 0).
 template <typename T>
-__device__ __forceinline__ T fminmd(T x, T y) {  // FMINMD, src/Functions.f90:10-26
-  const bool same = (x > T(0) & y > T(0)) | (x < T(0) & y < T(0));
-  const T m = (fabs(x) <= fabs(y)) ? x : y;
-  return same ? m : T(0);
+__device__ __forceinline__ T fminmd(T x, T y) {
+  return (copysign(T(0.5), x) + copysign(T(0.5), y)) * fmin(fabs(x), fabs(y));
 }
+// positive / negative parts (a+|a|)/2 and (a-|a|)/2 of the reference's
+// flux splitting, i.e. max(a,0) and min(a,0)
 template <typename T>
-__device__ __forceinline__ T posp(T a) { return (a + fabs(a)) * T(0.5); }
+__device__ __forceinline__ T posp(T a) { return fmax(a, T(0)); }
 template <typename T>
-__device__ __forceinline__ T negp(T a) { return (a - fabs(a)) * T(0.5); }
+__device__ __forceinline__ T negp(T a) { return fmin(a, T(0)); }
 
 // Block-wide sum of N doubles (wave64 shuffles, then one LDS pass).  All
 // threads receive the totals.  blockDim.x must be a multiple of 64, <= 1024.
@@ -461,8 +464,9 @@ __global__ void __launch_bounds__(256) k_sweep(Dev d, int VT, T *FG, T *FH, cons
 // ------------------------------------------------------------------------
 // Per-cell constants of the kinetic step, computed once per step so that the
 // per-velocity work needs no division, sorted by "wrapped anti-diagonal":
-// cell (i,j) goes to plane r = (i+j-2) mod (ny-1), slot i-1, as
-// cpar[r][i-1][k].  Each plane then
+// cell (i,j) goes to plane p = (i+j+2) mod (ny+3), slot i-1, as
+// cpar[p][i-1][k] (the fused kernel's position of row j in a tile is j+3,
+// after three virtual rows, and a tile has ny+3 positions).  Each plane then
 // holds exactly one cell per mesh column: the cells the fused kernel visits
 // in one step.  Values (k): 0 n/(pi T), 1 cpr/(n T^2) (Get_Feq,
 // src/GKUA_Residual.f90:449-450), 2 1/T, 3 T, 4 u, 5 v, 6 qx, 7 qy, 8 CF,
@@ -477,7 +481,7 @@ __global__ void k_cellpar(Dev d, double *cpar) {
   if (t >= ncol * nrow) return;
   const int i = t % ncol + 1, j = t / ncol + 1;
   const int c = cidx(d, i, j), N = d.ncell;
-  const int r = (i + j - 2) % nrow;
+  const int r = (i + j + 2) % (nrow + 4);
   double *o = cpar + ((size_t)r * ncol + (i - 1)) * kNW;
   const double n = d.S[c], T = d.S[3 * N + c];
   const int g0 = gidx(d, i, j), gi = gidx(d, i + 1, j), gj = gidx(d, i, j + 1);
@@ -612,7 +616,7 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
     for (int e = tid; e < nof; e += nthr)
       OF[e] = (e <= nrow + 1 ? ((d.ovrI1[e] ? 1 : 0) | (d.ovrI3[e] ? 2 : 0)) : 0) |
               (e < d.nx + 2 ? ((d.ovrJ2[e] ? 4 : 0) | (d.ovrJ4[e] ? 8 : 0)) : 0);
-    for (int e = tid; e < 2 * nconst; e += nthr) C[e] = cpar[nrow > 1 || e < nconst ? e : e - nconst];
+    for (int e = tid; e < 2 * nconst; e += nthr) C[e] = cpar[e];  // planes 0 and 1
   }
   // Per-thread indices are recomputed inside the loop from a laundered copy
   // of threadIdx.x: keeping them live across the loop costs more registers
@@ -629,54 +633,31 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
   auto soff = [&](unsigned face, unsigned pos, unsigned ird, unsigned v) {
     return slotb + (((face - 1) * L2 + 2 * pos + ird) * (unsigned)nv + v) * ES;
   };
-  // values of the next position, loaded one step ahead; n0..n3 hold
-  // F(i,-1..2) of the next tile, which arrive one row per step (via pn, for
-  // slot pm) during the last four rows of a tile
-  T n0 = T(0), n1 = T(0), n2 = T(0), n3 = T(0), pw = T(0), pm1 = T(0), pp1 = T(0), pp2 = T(0), px = T(0), py = T(0);
-  T pn = T(0);
-  int pm = -1;
-  // loads of position (kn, jn) of column i: the stencil values beside the
-  // window and the wall-slot / ghost values of the boundary faces (addresses
-  // that a column does not need point at its own cell); returns the offset
-  // of cell (i, jn)
-  auto load_pos = [&](int i, int lane, int kn, int jn) {
-    unsigned tb;
-    int ird, vv;
-    tile_of(kn, lane, tb, ird, vv);
-    const unsigned oc = tb + (unsigned)(jn + 1) * rb + (unsigned)(i + 1) * cb;
-    const int rf = OF[jn], cf = OF[i] >> 2;
-    const unsigned ax = i == 1 ? ((rf & 1) ? soff(1, jn + 1, ird, vv) : oc - 2 * cb)
-                               : ((i == ncol && (rf & 2)) ? soff(3, jn + 1, ird, vv) : oc);
-    const unsigned ay = (jn == 1 && (cf & 1)) ? soff(2, i + 1, ird, vv)
-                                              : ((jn == nrow && (cf & 2)) ? soff(4, i + 1, ird, vv) : oc);
-    pw = ldo(F, oc + 2 * rb);
-    pm1 = ldo(F, oc - cb);
-    pp1 = ldo(F, oc + cb);
-    pp2 = ldo(F, oc + 2 * cb);
-    px = ldo(F, ax);
-    py = ldo(F, ay);
-    return oc;
-  };
+  // Each tile's sequence has nrow+4 positions: four "virtual" rows -3..0,
+  // which only shift F(i,-1..2) of the tile into the register window (and
+  // finalise the previous tile's last two rows), then rows 1..nrow.  The
+  // window shift is then the same at every step and the value to finalise is
+  // always the oldest window entry.
+  const int LT = nrow + 4;
+  const int nseq = ktiles * LT;
+  const int tsteps = (ncol - 1) + nseq + 2;
+  // values of the next position, loaded one step ahead
+  T pw = T(0), pm1 = T(0), pp1 = T(0), pp2 = T(0), px = T(0), py = T(0);
   __syncthreads();  // OF, V and the first two constants planes
   {
     const int tl = threadIdx.x, lane = tl & (VT - 1), i = (tl >> vshift) + 1;
-    const unsigned oc = load_pos(i, lane, 0, 1);
-    n0 = ldo(F, oc - 2 * rb);
-    n1 = ldo(F, oc - rb);
-    n2 = ldo(F, oc);
-    n3 = ldo(F, oc + rb);
+    unsigned tb;
+    int ird, vv;
+    tile_of(0, lane, tb, ird, vv);
+    pw = ldo(F, tb + (unsigned)(i + 1) * cb);  // F(i, -1) of the first tile
   }
-
-  T w0 = T(0), w1 = T(0), w2 = T(0), w3 = T(0), w4 = T(0);  // F(i, j-2..j+2)
-  T wp0 = T(0), wp1 = T(0);  // F(i, ny-2), F(i, ny-1) of the previous tile
+  T w0 = T(0), w1 = T(0), w2 = T(0), w3 = T(0), w4 = T(0);  // F(i, r-2..r+2)
   T fly_prev = T(0);
-  int jc = 0, kc = 0;  // row and tile of the current position
-  const int nseq = ktiles * nrow;
-  const int tsteps = (ncol - 1) + nseq + 2;
+  int r = -4, kc = 0;  // row and tile of the current position
   // The constants of step s are loaded at step s-2 and written to LDS buffer
   // s&1 at the top of step s-1, after that step's wait for the loads issued
   // one step earlier; planes 0 and 1 are in place before the loop.
-  int rstage = 2 % nrow;
+  int rstage = 2 % LT;
   double stg[3] = {0.0, 0.0, 0.0};
   for (int t = 0; t < tsteps; t++) {
     int tl = threadIdx.x;
@@ -694,61 +675,43 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
 #pragma unroll
       for (int k = 0; k < 3; k++) stg[k] = src[min(tl + k * nthr, nconst - 1)];
     }
-    rstage = rstage + 1 == nrow ? 0 : rstage + 1;
-    // the next tile's window: slot pm receives the value loaded one step ago
-    n0 = pm == 0 ? pn : n0;
-    n1 = pm == 1 ? pn : n1;
-    n2 = pm == 2 ? pn : n2;
-    n3 = pm == 3 ? pn : n3;
+    rstage = rstage + 1 == LT ? 0 : rstage + 1;
     const bool active = q >= 0 && q < nseq;
     if (active) {
-      if (q == 0) { jc = 1; kc = 0; }
-      else if (jc == nrow) { jc = 1; kc++; }
-      else jc++;
+      if (++r > nrow) {
+        r = -3;
+        kc++;
+      }
     }
-    const int j = jc;
-    const bool first = j == 1;
-    // this position's inputs (loaded one step ago), then the next position's loads
-    if (active && first) { wp0 = w1; wp1 = w2; }
-    const T fm1 = pm1, fp1 = pp1, fp2 = pp2, xs = px, ys = py;
-    if (active) {
-      w0 = first ? n0 : w1;
-      w1 = first ? n1 : w2;
-      w2 = first ? n2 : w3;
-      w3 = first ? n3 : w4;
+    if (q >= 0 && q < nseq + 2) {
+      w0 = w1;
+      w1 = w2;
+      w2 = w3;
+      w3 = w4;
       w4 = pw;
     }
-    // Loads of the next position, issued unconditionally (addresses that are
-    // not needed point at the position's own cell) so that the only wait for
-    // them is at the top of the next step.  Before a column starts, the next
-    // position is its first one; after its last, the loads are not used.
-    {
-      const bool started = q >= 0;
-      const bool nvalid = active && q + 1 < nseq;
-      const bool wrap = j == nrow;
-      const int kn = !started ? 0 : (nvalid && wrap ? kc + 1 : kc);
-      const int jn = !started ? 1 : (nvalid ? (wrap ? 1 : j + 1) : max(j, 1));
-      const unsigned oc = load_pos(i, lane, kn, jn);
-      // rows nrow-3..nrow of a tile fetch rows -1..2 of the next tile
-      const int m = j - (nrow - 3);
-      const bool wantn = nvalid && m >= 0 && kc + 1 < ktiles;
-      unsigned tb1;
-      int ird1, vv1;
-      tile_of(kc + 1, lane, tb1, ird1, vv1);
-      pm = wantn ? m : -1;
-      pn = ldo(F, wantn ? tb1 + (unsigned)m * rb + (unsigned)(i + 1) * cb : oc);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    if (active) {
+    const int h1 = (t + 2) % 3, h2 = (t + 1) % 3, hs = t % 3;  // ring slots of q-1, q-2, q
+    // Interior fast path, taken when every lane of the wavefront is at an
+    // interior position (no face override, no wall slot or ghost value
+    // within reach, finalised row in the same tile): plain NND fluxes and no
+    // boundary loads.  Everything else takes the general path.
+    const bool inner = active && r >= 3 && r <= nrow - 2 && i >= 2 && i <= ncol - 2;
+    if (__all(inner)) {
+      const T fm1 = pm1, fp1 = pp1, fp2 = pp2;
       unsigned tb;
       int ird, vv;
       const bool act = tile_of(kc, lane, tb, ird, vv);
+      const unsigned oc = tb + (unsigned)(r + 1) * rb + (unsigned)(i + 1) * cb;  // cell (i, r)
+      pw = ldo(F, oc + 3 * rb);
+      pm1 = ldo(F, oc + rb - cb);
+      pp1 = ldo(F, oc + rb + cb);
+      pp2 = ldo(F, oc + rb + 2 * cb);
+      __builtin_amdgcn_sched_barrier(0);
       const double2 vxy = V[kc * VT + lane];
       const T vx = T(vxy.x), vy = T(vxy.y);
       const double *cc = C + (buf * ncol + col) * kNW;
-      // Shakhov equilibrium of cell (i,j) (Get_Feq, src/GKUA_Residual.f90:420-469)
       T feq;
-      {
+      {  // Shakhov equilibrium of cell (i,r) (Get_Feq, src/GKUA_Residual.f90:420-469)
         const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
         const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
         const T fvx = T(cc[1]) * (viu * T(cc[6]) + vjv * T(cc[7]));
@@ -756,56 +719,147 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
         feq = ird == 0 ? gd * (T(1) + fvx * (T(2) * vij2 - T(4)))
                        : (T(cc[3]) * gd / T(2)) * (T(1) + fvx * (T(2) * vij2 - T(2)));
       }
-      const int rf = OF[j], cf = OF[i] >> 2;
       const T axl = vx * T(cc[12]) + vy * T(cc[13]);
       const T axr = vx * T(cc[14]) + vy * T(cc[15]);
-      T flxl, dl = T(0);
-      if (i == 1) {
-        flxl = wflux(1, nx, rf, axl, xs, fm1, w2, fp1, xs, nnd);
-      } else {
-        const T2 xl = X[(pbuf * ncol + col - 1) * VT + lane];
-        flxl = xl.x;
-        dl = xl.y;
-      }
-      const T flxr = wflux(i + 1, nx, rf, axr, fm1, w2, fp1, fp2, xs, nnd);
       const T ayb = vx * T(cc[16]) + vy * T(cc[17]);
       const T ayt = vx * T(cc[18]) + vy * T(cc[19]);
-      const T flyb = first ? wflux(1, ny, cf, ayb, w0, w1, w2, w3, ys, nnd) : fly_prev;
-      const T flyt = wflux(j + 1, ny, cf, ayt, w1, w2, w3, w4, ys, nnd);
+      const T2 xl = X[(pbuf * ncol + col - 1) * VT + lane];
+      const T flxr = flux_nnd(axr, fm1, w2, fp1, fp2, nnd);
+      const T flyt = flux_nnd(ayt, w1, w2, w3, w4, nnd);
+      const T rr = -(((flxr - xl.x) + flyt) - fly_prev) * T(cc[9]) + T(cc[8]) * (feq - w2);
       fly_prev = flyt;
-      const T r = -(((flxr - flxl) + flyt) - flyb) * T(cc[9]) + T(cc[8]) * (feq - w2);
-      if (store_rhs && act) sto(R, tb + (unsigned)(j + 1) * rb + (unsigned)(i + 1) * cb, r);
-      // forward recursion (src/GKUA_Residual.f90:39-57); Delta*(i,0) = Delta*(0,j) = 0
+      if (store_rhs && act) sto(R, oc, rr);
+      // forward recursion (src/GKUA_Residual.f90:39-57)
       const T alfa = T(cc[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
       const T ra = T(1) / alfa;
-      const int hs = t % 3;  // history ring slot of position q
-      const T db = first ? T(0) : H[(((t + 2) % 3) * 3) * nthr + tl];
-      const T ds = (T(cc[10]) * r + posp(axl) * dl + posp(ayb) * db) * ra;
+      const T db = H[(h1 * 3) * nthr + tl];
+      const T ds = (T(cc[10]) * rr + posp(axl) * xl.y + posp(ayb) * db) * ra;
       T2 xo;
       xo.x = flxr;
       xo.y = ds;
       X[(buf * ncol + col) * VT + lane] = xo;
+      // backward pass + update of row r-2 (src/GKUA_Residual.f90:64-109);
+      // Delta*(i, r-1) = db
+      const T dr = X[(pbuf * ncol + col + 1) * VT + lane].y;
+      const T du = H[(h2 * 3) * nthr + tl] + (H[(h2 * 3 + 1) * nthr + tl] * dr + H[(h2 * 3 + 2) * nthr + tl] * db);
       H[(hs * 3 + 0) * nthr + tl] = ds;
       H[(hs * 3 + 1) * nthr + tl] = -negp(axr) * ra;
       H[(hs * 3 + 2) * nthr + tl] = -negp(ayt) * ra;
-    }
-    // backward pass + update of position q-2 (src/GKUA_Residual.f90:64-109)
-    if (q >= 2 && q < nseq + 2) {
-      const bool live = q < nseq;
-      const int jf = live ? (jc > 2 ? jc - 2 : jc - 2 + nrow) : nrow - 1 + (q - nseq);
-      const bool prev = live && jc <= 2;
-      const int h2 = (t + 1) % 3, h1 = (t + 2) % 3;  // slots of positions q-2 and q-1
-      const T dr = (i < ncol) ? X[(pbuf * ncol + col + 1) * VT + lane].y : T(0);
-      // Delta*(i, jf+1) is the value of position q-1, 0 past the last row
-      const T dn = (jf == nrow) ? T(0) : H[(h1 * 3) * nthr + tl];
-      const T du = H[(h2 * 3) * nthr + tl] + (H[(h2 * 3 + 1) * nthr + tl] * dr + H[(h2 * 3 + 2) * nthr + tl] * dn);
-      const T fold = prev ? (jc == 1 ? wp0 : wp1) : (live ? w0 : (q == nseq ? w1 : w2));
-      unsigned tb;
-      int ird, vv;
-      if (tile_of(prev ? kc - 1 : kc, lane, tb, ird, vv)) {
-        T f = fold + du;
+      if (act) {
+        T f = w0 + du;
         if ((double)f < kClamp) f = T(kClamp);
-        sto(F, tb + (unsigned)(jf + 1) * rb + (unsigned)(i + 1) * cb, f);
+        sto(F, oc - 2 * rb, f);
+      }
+    } else {
+      const T fm1 = pm1, fp1 = pp1, fp2 = pp2, xs = px, ys = py;
+      // Loads of the next position, issued unconditionally (addresses that
+      // are not needed point at the cell itself) so that the only wait for
+      // them is at the top of the next step.  Before a column starts the next
+      // position is its first one; after its last the loads are not used.
+      {
+        const bool started = q >= 0;
+        const bool nvalid = active && q + 1 < nseq;
+        const int rn = !started ? -3 : (nvalid ? (r == nrow ? -3 : r + 1) : nrow);
+        const int kn = !started ? 0 : (nvalid && r == nrow ? kc + 1 : kc);
+        unsigned tb;
+        int ird, vv;
+        tile_of(kn, lane, tb, ird, vv);
+        // cell (i, rn) (row rn >= -3 may lie below the padded grid; its
+        // offset is only used for rows >= 1) and cell (i, rn+2) (row >= -1)
+        const unsigned oc = tb + (unsigned)(rn + 1) * rb + (unsigned)(i + 1) * cb;
+        const unsigned ow = oc + 2 * rb;
+        const bool rreal = rn >= 1;
+        const int rf = OF[rreal ? rn : 1], cf = OF[i] >> 2;
+        const unsigned ax = !rreal ? ow
+                                   : (i == 1 ? ((rf & 1) ? soff(1, rn + 1, ird, vv) : oc - 2 * cb)
+                                             : ((i == ncol && (rf & 2)) ? soff(3, rn + 1, ird, vv) : ow));
+        const unsigned ay = (rn == 1 && (cf & 1)) ? soff(2, i + 1, ird, vv)
+                                                  : ((rn == nrow && (cf & 2)) ? soff(4, i + 1, ird, vv) : ow);
+        pw = ldo(F, ow);
+        pm1 = ldo(F, rreal ? oc - cb : ow);
+        pp1 = ldo(F, rreal ? oc + cb : ow);
+        pp2 = ldo(F, rreal ? oc + 2 * cb : ow);
+        px = ldo(F, ax);
+        py = ldo(F, ay);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if (active && r >= 1) {
+        const bool first = r == 1;
+        unsigned tb;
+        int ird, vv;
+        const bool act = tile_of(kc, lane, tb, ird, vv);
+        const double2 vxy = V[kc * VT + lane];
+        const T vx = T(vxy.x), vy = T(vxy.y);
+        const double *cc = C + (buf * ncol + col) * kNW;
+        T feq;
+        {  // Shakhov equilibrium of cell (i,r) (Get_Feq, src/GKUA_Residual.f90:420-469)
+          const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
+          const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
+          const T fvx = T(cc[1]) * (viu * T(cc[6]) + vjv * T(cc[7]));
+          const T gd = T(cc[0]) * wexp(-vij2);
+          feq = ird == 0 ? gd * (T(1) + fvx * (T(2) * vij2 - T(4)))
+                         : (T(cc[3]) * gd / T(2)) * (T(1) + fvx * (T(2) * vij2 - T(2)));
+        }
+        const int rf = OF[r], cf = OF[i] >> 2;
+        const T axl = vx * T(cc[12]) + vy * T(cc[13]);
+        const T axr = vx * T(cc[14]) + vy * T(cc[15]);
+        T flxl, dl = T(0);
+        if (i == 1) {
+          flxl = wflux(1, nx, rf, axl, xs, fm1, w2, fp1, xs, nnd);
+        } else {
+          const T2 xl = X[(pbuf * ncol + col - 1) * VT + lane];
+          flxl = xl.x;
+          dl = xl.y;
+        }
+        const T flxr = wflux(i + 1, nx, rf, axr, fm1, w2, fp1, fp2, xs, nnd);
+        const T ayb = vx * T(cc[16]) + vy * T(cc[17]);
+        const T ayt = vx * T(cc[18]) + vy * T(cc[19]);
+        const T flyb = first ? wflux(1, ny, cf, ayb, w0, w1, w2, w3, ys, nnd) : fly_prev;
+        const T flyt = wflux(r + 1, ny, cf, ayt, w1, w2, w3, w4, ys, nnd);
+        fly_prev = flyt;
+        const T rr = -(((flxr - flxl) + flyt) - flyb) * T(cc[9]) + T(cc[8]) * (feq - w2);
+        if (store_rhs && act) sto(R, tb + (unsigned)(r + 1) * rb + (unsigned)(i + 1) * cb, rr);
+        // forward recursion (src/GKUA_Residual.f90:39-57); Delta*(i,0) = Delta*(0,j) = 0
+        const T alfa = T(cc[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
+        const T ra = T(1) / alfa;
+        const T db = first ? T(0) : H[(h1 * 3) * nthr + tl];
+        const T ds = (T(cc[10]) * rr + posp(axl) * dl + posp(ayb) * db) * ra;
+        T2 xo;
+        xo.x = flxr;
+        xo.y = ds;
+        X[(buf * ncol + col) * VT + lane] = xo;
+        H[(hs * 3 + 0) * nthr + tl] = ds;
+        H[(hs * 3 + 1) * nthr + tl] = -negp(axr) * ra;
+        H[(hs * 3 + 2) * nthr + tl] = -negp(ayt) * ra;
+      }
+      // backward pass + update of position q-2 (src/GKUA_Residual.f90:64-109)
+      if (q >= 2 && q < nseq + 2) {
+        int jf, kf;
+        if (q < nseq) {
+          jf = r - 2;
+          kf = kc;
+          if (jf < -3) {
+            jf += LT;
+            kf = kc - 1;
+          }
+        } else {
+          jf = nrow - 1 + (q - nseq);
+          kf = ktiles - 1;
+        }
+        if (jf >= 1) {
+          const T dr = (i < ncol) ? X[(pbuf * ncol + col + 1) * VT + lane].y : T(0);
+          // Delta*(i, jf+1) is the value of position q-1, 0 past the last row
+          const T dn = (jf == nrow) ? T(0) : H[(h1 * 3) * nthr + tl];
+          const T du =
+              H[(h2 * 3) * nthr + tl] + (H[(h2 * 3 + 1) * nthr + tl] * dr + H[(h2 * 3 + 2) * nthr + tl] * dn);
+          unsigned tb;
+          int ird, vv;
+          if (tile_of(kf, lane, tb, ird, vv)) {
+            T f = w0 + du;
+            if ((double)f < kClamp) f = T(kClamp);
+            sto(F, tb + (unsigned)(jf + 1) * rb + (unsigned)(i + 1) * cb, f);
+          }
+        }
       }
     }
     __syncthreads();
