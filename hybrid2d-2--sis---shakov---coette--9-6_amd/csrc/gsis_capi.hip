@@ -283,7 +283,7 @@ int gsis_create(const gsis_params *p, gsis_handle **out) {
       (rc = dalloc(h, &d.taoi, 8 * 4 * N)) || (rc = dalloc(h, &d.taoj, 8 * 4 * N)) ||
       (rc = dalloc(h, &d.qi, 8 * 2 * N)) || (rc = dalloc(h, &d.qj, 8 * 2 * N)) ||
       (rc = dalloc(h, &d.prim, 8 * 5 * N)) || (rc = dalloc(h, &d.res4, 8 * 4 * N)) || (rc = dalloc(h, &d.err, 64)) ||
-      (rc = dalloc(h, &h->cpar, 8 * gsis::kNCellPar * N)) ||
+      (rc = dalloc(h, &h->cpar, 8 * (size_t)(ny + 3) * 3 * 1024)) ||  // ny+3 planes of <= 3*1024 constants
       (rc = dalloc(h, &h->resbuf, 8 * 8)))
     return cleanup(rc);
   {  // Creat_Mesh sets U, U0, U2 to (1,0,0,1) (src/sub_init.f90:159-162)

@@ -15,12 +15,10 @@ namespace gsis {
 // small helpers
 // ------------------------------------------------------------------------
 // FMINMD (src/Functions.f90:10-26): the argument of smaller magnitude when
-// both have the same sign, else 0.  Written as (sign(x)+sign(y))/2 * min(|x|,|y|),
-// which gives the same values without branches (a zero argument gives 0).
-template <typename T>
-__device__ __forceinline__ T fminmd(T x, T y) {
-  return (copysign(T(0.5), x) + copysign(T(0.5), y)) * fmin(fabs(x), fabs(y));
-}
+// both have the same sign, else 0 -- the median of (x, y, 0), written without
+// branches (v_med3_f32 in FP32).
+__device__ __forceinline__ double fminmd(double x, double y) { return fmax(fmin(x, y), fmin(fmax(x, y), 0.0)); }
+__device__ __forceinline__ float fminmd(float x, float y) { return __builtin_amdgcn_fmed3f(x, y, 0.0f); }
 // positive / negative parts (a+|a|)/2 and (a-|a|)/2 of the reference's
 // flux splitting, i.e. max(a,0) and min(a,0)
 template <typename T>
@@ -465,7 +463,7 @@ __global__ void __launch_bounds__(256) k_sweep(Dev d, int VT, T *FG, T *FH, cons
 // Per-cell constants of the kinetic step, computed once per step so that the
 // per-velocity work needs no division, sorted by "wrapped anti-diagonal":
 // cell (i,j) goes to plane p = (i+j+2) mod (ny+3), slot i-1, as
-// cpar[p][i-1][k] (the fused kernel's position of row j in a tile is j+3,
+// cpar[p*pstride + (i-1)*kNW + k] (the fused kernel's position of row j in a tile is j+3,
 // after three virtual rows, and a tile has ny+3 positions).  Each plane then
 // holds exactly one cell per mesh column: the cells the fused kernel visits
 // in one step.  Values (k): 0 n/(pi T), 1 cpr/(n T^2) (Get_Feq,
@@ -475,14 +473,14 @@ __global__ void __launch_bounds__(256) k_sweep(Dev d, int VT, T *FG, T *FH, cons
 // 12-13 i-face i, 14-15 i-face i+1, 16-17 j-face j, 18-19 j-face j+1.
 // ------------------------------------------------------------------------
 constexpr int kNW = kNCellPar;
-__global__ void k_cellpar(Dev d, double *cpar) {
+__global__ void k_cellpar(Dev d, double *cpar, int pstride) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ncol = d.nx - 1, nrow = d.ny - 1;
   if (t >= ncol * nrow) return;
   const int i = t % ncol + 1, j = t / ncol + 1;
   const int c = cidx(d, i, j), N = d.ncell;
   const int r = (i + j + 2) % (nrow + 4);
-  double *o = cpar + ((size_t)r * ncol + (i - 1)) * kNW;
+  double *o = cpar + (size_t)r * pstride + (size_t)(i - 1) * kNW;
   const double n = d.S[c], T = d.S[3 * N + c];
   const int g0 = gidx(d, i, j), gi = gidx(d, i + 1, j), gj = gidx(d, i, j + 1);
   const double vol = d.vol[g0], cf = d.CF[c];
@@ -500,20 +498,23 @@ __global__ void k_cellpar(Dev d, double *cpar) {
 // sv is the wall-slot value of the boundary face.  Next to an overridden face
 // the limiter on that side is dropped (flux_lowadj / flux_highadj), on it the
 // flux is a * slot (src/GKUA_Residual.f90:200-262).
+// hs is 1/2 for the NND scheme and 0 for first-order upwind (the limited
+// slopes then drop out exactly).
 template <typename T>
-__device__ __forceinline__ T wflux(int f, int n, int fl, T a, T fm2, T fm1, T f0, T fp1, T sv, bool nnd) {
+__device__ __forceinline__ T wflux(int f, int n, int fl, T a, T fm2, T fm1, T f0, T fp1, T sv, T hs) {
   const T asp = posp(a), asn = negp(a);
-  T ul = fm1, ur = f0;
-  if (nnd) {
-    const T d0 = f0 - fm1;
-    const T liml = ((fl & 1) && f == 2) ? d0 : fminmd(d0, fm1 - fm2);
-    const T limr = ((fl & 2) && f == n - 1) ? d0 : fminmd(d0, fp1 - f0);
-    ul = fm1 + T(0.5) * liml;
-    ur = f0 - T(0.5) * limr;
-  }
-  const T fx = asp * ul + asn * ur;
+  const T d0 = f0 - fm1;
+  const T liml = ((fl & 1) && f == 2) ? d0 : fminmd(d0, fm1 - fm2);
+  const T limr = ((fl & 2) && f == n - 1) ? d0 : fminmd(d0, fp1 - f0);
+  const T fx = asp * (fm1 + hs * liml) + asn * (f0 - hs * limr);
   const bool wall = ((fl & 1) && f == 1) || ((fl & 2) && f == n);
   return wall ? a * sv : fx;
+}
+// interior face (no override) of the fused kernel's fast path
+template <typename T>
+__device__ __forceinline__ T iflux(T a, T fm2, T fm1, T f0, T fp1, T hs) {
+  const T d0 = f0 - fm1;
+  return posp(a) * (fm1 + hs * fminmd(d0, fm1 - fm2)) + negp(a) * (f0 - hs * fminmd(d0, fp1 - f0));
 }
 // exp(x) for the equilibrium, x <= 0.  Cody-Waite reduction x = k ln2 + r
 // (|r| <= ln2/2, ln2 split into the fdlibm high/low parts so that k*ln2_hi is
@@ -538,12 +539,32 @@ __device__ __forceinline__ double wexp(double x) {
   const double k = rint(x * kc[12]);
   double r = fma(k, kc[13], x);
   r = fma(k, kc[14], r);
+  // Horner steps p <- p*r + c with the coefficient read straight from its
+  // scalar register (the compiler would copy each one into a vector register
+  // pair first to use the accumulating fma form)
   double p = kc[0];
 #pragma unroll
-  for (int m = 1; m < 12; m++) p = fma(p, r, kc[m]);
+  for (int m = 1; m < 12; m++) {
+    const double c = kc[m];
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(p) : "v"(p), "v"(r), "s"(c));
+  }
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
   return ldexp(p, (int)k);
+}
+// 1/a for a > 0 (the LU-SGS diagonal): hardware reciprocal estimate and two
+// Newton-Raphson steps r <- r + r(1 - a r), accurate to about 1 ulp
+__device__ __forceinline__ double rcp_pos(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = fma(-a, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-a, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_pos(float a) {
+  float r = __builtin_amdgcn_rcpf(a);
+  const float e = fmaf(-a, r, 1.0f);
+  return fmaf(r, e, r);
 }
 __device__ __forceinline__ float wexp(float x) { return expf(x); }
 // element at a 32-bit byte offset from a uniform base (global_load with an
@@ -591,8 +612,10 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
   extern __shared__ double lds_d[];
   const int ncol = d.nx - 1, nrow = d.ny - 1, nv = d.nv, nx = d.nx, ny = d.ny;
   const int nthr = blockDim.x;
-  const int nconst = kNW * ncol;
-  double *C = lds_d;                                            // [2][ncol][kNW]
+  // one plane of cell constants, padded to 3 values per thread so that the
+  // cooperative copy needs no bounds test (kNW*ncol <= 3*nthr since VT >= 8)
+  const int nconst = 3 * nthr;
+  double *C = lds_d;                                            // [2][nconst]: [ncol][kNW] + padding
   double2 *V = reinterpret_cast<double2 *>(C + 2 * nconst);     // [K][VT] velocities of the tiles
   T2 *X = reinterpret_cast<T2 *>(V + K * VT);                   // [2][ncol][VT] (face flux, Delta*)
   T *H = reinterpret_cast<T *>(X + 2 * ncol * VT);              // [3][3][nthr] history ring
@@ -600,7 +623,7 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
   const int ntiles = 2 * ((nv + VT - 1) / VT);
   const int tile0 = blockIdx.x * K;
   const int ktiles = min(K, ntiles - tile0);
-  const bool nnd = d.scheme != 0;
+  const T hs = d.scheme != 0 ? T(0.5) : T(0);  // NND slope factor (0: upwind)
   const unsigned cb = (unsigned)nv * ES;       // one cell
   const unsigned rb = (unsigned)d.NI * cb;     // one row of cells
   const unsigned L2 = 2u * d.L;
@@ -660,6 +683,10 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
   int rstage = 2 % LT;
   double stg[3] = {0.0, 0.0, 0.0};
   for (int t = 0; t < tsteps; t++) {
+    // Everything issued in the previous step (its loads for this step, the
+    // stage loads, its store) is complete from here on: one explicit wait,
+    // so that no later instruction of the step waits for memory.
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     int tl = threadIdx.x;
     asm volatile("" : "+v"(tl));
     const int lane = tl & (VT - 1), col = tl >> vshift, i = col + 1;
@@ -667,14 +694,10 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
     const int q = t - col;
     if (t > 0) {
 #pragma unroll
-      for (int k = 0; k < 3; k++)
-        if (tl + k * nthr < nconst) C[pbuf * nconst + tl + k * nthr] = stg[k];
+      for (int k = 0; k < 3; k++) C[pbuf * nconst + k * nthr + tl] = stg[k];
     }
-    {
-      const double *src = cpar + (size_t)rstage * nconst;
 #pragma unroll
-      for (int k = 0; k < 3; k++) stg[k] = src[min(tl + k * nthr, nconst - 1)];
-    }
+    for (int k = 0; k < 3; k++) stg[k] = ldo(cpar + (size_t)rstage * nconst + k * nthr, (unsigned)tl * 8u);
     rstage = rstage + 1 == LT ? 0 : rstage + 1;
     const bool active = q >= 0 && q < nseq;
     if (active) {
@@ -683,14 +706,44 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
         kc++;
       }
     }
-    if (q >= 0 && q < nseq + 2) {
-      w0 = w1;
-      w1 = w2;
-      w2 = w3;
-      w3 = w4;
-      w4 = pw;
+    // (before a column starts and after it ends the window holds unused values)
+    w0 = w1;
+    w1 = w2;
+    w2 = w3;
+    w3 = w4;
+    w4 = pw;
+    const int h1 = (t + 2) % 3, h2 = (t + 1) % 3, hsl = t % 3;  // ring slots of q-1, q-2, q
+    // Backward pass + update of position q-2 (src/GKUA_Residual.f90:64-109),
+    // first thing in the step: everything it reads was written before the
+    // last barrier, and its store then has the whole step to complete.
+    if (q >= 2 && q < nseq + 2) {
+      int jf, kf;
+      if (q < nseq) {
+        jf = r - 2;
+        kf = kc;
+        if (jf < -3) {
+          jf += LT;
+          kf = kc - 1;
+        }
+      } else {
+        jf = nrow - 1 + (q - nseq);
+        kf = ktiles - 1;
+      }
+      if (jf >= 1) {
+        const T dr = (i < ncol) ? X[(pbuf * ncol + col + 1) * VT + lane].y : T(0);
+        // Delta*(i, jf+1) is the value of position q-1, 0 past the last row
+        const T dn = (jf == nrow) ? T(0) : H[(h1 * 3) * nthr + tl];
+        const T du =
+            H[(h2 * 3) * nthr + tl] + (H[(h2 * 3 + 1) * nthr + tl] * dr + H[(h2 * 3 + 2) * nthr + tl] * dn);
+        unsigned tb;
+        int ird, vv;
+        if (tile_of(kf, lane, tb, ird, vv)) {
+          T f = w0 + du;
+          if ((double)f < kClamp) f = T(kClamp);
+          sto(F, tb + (unsigned)(jf + 1) * rb + (unsigned)(i + 1) * cb, f);
+        }
+      }
     }
-    const int h1 = (t + 2) % 3, h2 = (t + 1) % 3, hs = t % 3;  // ring slots of q-1, q-2, q
     // Interior fast path, taken when every lane of the wavefront is at an
     // interior position (no face override, no wall slot or ghost value
     // within reach, finalised row in the same tile): plain NND fluxes and no
@@ -709,47 +762,39 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
       __builtin_amdgcn_sched_barrier(0);
       const double2 vxy = V[kc * VT + lane];
       const T vx = T(vxy.x), vy = T(vxy.y);
-      const double *cc = C + (buf * ncol + col) * kNW;
+      const double *cc = C + buf * nconst + col * kNW;
       T feq;
       {  // Shakhov equilibrium of cell (i,r) (Get_Feq, src/GKUA_Residual.f90:420-469)
         const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
         const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
         const T fvx = T(cc[1]) * (viu * T(cc[6]) + vjv * T(cc[7]));
         const T gd = T(cc[0]) * wexp(-vij2);
-        feq = ird == 0 ? gd * (T(1) + fvx * (T(2) * vij2 - T(4)))
-                       : (T(cc[3]) * gd / T(2)) * (T(1) + fvx * (T(2) * vij2 - T(2)));
+        // G: gd (1 + phi (2c^2 - 4)); H: (T gd / 2)(1 + phi (2c^2 - 2))
+        const T pre = ird == 0 ? gd : (T(cc[3]) * gd) * T(0.5);
+        feq = pre * (T(1) + fvx * (T(2) * vij2 - (ird == 0 ? T(4) : T(2))));
       }
       const T axl = vx * T(cc[12]) + vy * T(cc[13]);
       const T axr = vx * T(cc[14]) + vy * T(cc[15]);
       const T ayb = vx * T(cc[16]) + vy * T(cc[17]);
       const T ayt = vx * T(cc[18]) + vy * T(cc[19]);
       const T2 xl = X[(pbuf * ncol + col - 1) * VT + lane];
-      const T flxr = flux_nnd(axr, fm1, w2, fp1, fp2, nnd);
-      const T flyt = flux_nnd(ayt, w1, w2, w3, w4, nnd);
+      const T flxr = iflux(axr, fm1, w2, fp1, fp2, hs);
+      const T flyt = iflux(ayt, w1, w2, w3, w4, hs);
       const T rr = -(((flxr - xl.x) + flyt) - fly_prev) * T(cc[9]) + T(cc[8]) * (feq - w2);
       fly_prev = flyt;
-      if (store_rhs && act) sto(R, oc, rr);
       // forward recursion (src/GKUA_Residual.f90:39-57)
       const T alfa = T(cc[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
-      const T ra = T(1) / alfa;
+      const T ra = rcp_pos(alfa);
       const T db = H[(h1 * 3) * nthr + tl];
       const T ds = (T(cc[10]) * rr + posp(axl) * xl.y + posp(ayb) * db) * ra;
       T2 xo;
       xo.x = flxr;
       xo.y = ds;
       X[(buf * ncol + col) * VT + lane] = xo;
-      // backward pass + update of row r-2 (src/GKUA_Residual.f90:64-109);
-      // Delta*(i, r-1) = db
-      const T dr = X[(pbuf * ncol + col + 1) * VT + lane].y;
-      const T du = H[(h2 * 3) * nthr + tl] + (H[(h2 * 3 + 1) * nthr + tl] * dr + H[(h2 * 3 + 2) * nthr + tl] * db);
-      H[(hs * 3 + 0) * nthr + tl] = ds;
-      H[(hs * 3 + 1) * nthr + tl] = -negp(axr) * ra;
-      H[(hs * 3 + 2) * nthr + tl] = -negp(ayt) * ra;
-      if (act) {
-        T f = w0 + du;
-        if ((double)f < kClamp) f = T(kClamp);
-        sto(F, oc - 2 * rb, f);
-      }
+      H[(hsl * 3 + 0) * nthr + tl] = ds;
+      H[(hsl * 3 + 1) * nthr + tl] = -negp(axr) * ra;
+      H[(hsl * 3 + 2) * nthr + tl] = -negp(ayt) * ra;
+      if (store_rhs && act) sto(R, oc, rr);
     } else {
       const T fm1 = pm1, fp1 = pp1, fp2 = pp2, xs = px, ys = py;
       // Loads of the next position, issued unconditionally (addresses that
@@ -790,76 +835,47 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
         const bool act = tile_of(kc, lane, tb, ird, vv);
         const double2 vxy = V[kc * VT + lane];
         const T vx = T(vxy.x), vy = T(vxy.y);
-        const double *cc = C + (buf * ncol + col) * kNW;
+        const double *cc = C + buf * nconst + col * kNW;
         T feq;
         {  // Shakhov equilibrium of cell (i,r) (Get_Feq, src/GKUA_Residual.f90:420-469)
           const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
           const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
           const T fvx = T(cc[1]) * (viu * T(cc[6]) + vjv * T(cc[7]));
           const T gd = T(cc[0]) * wexp(-vij2);
-          feq = ird == 0 ? gd * (T(1) + fvx * (T(2) * vij2 - T(4)))
-                         : (T(cc[3]) * gd / T(2)) * (T(1) + fvx * (T(2) * vij2 - T(2)));
+          const T pre = ird == 0 ? gd : (T(cc[3]) * gd) * T(0.5);
+          feq = pre * (T(1) + fvx * (T(2) * vij2 - (ird == 0 ? T(4) : T(2))));
         }
         const int rf = OF[r], cf = OF[i] >> 2;
         const T axl = vx * T(cc[12]) + vy * T(cc[13]);
         const T axr = vx * T(cc[14]) + vy * T(cc[15]);
         T flxl, dl = T(0);
         if (i == 1) {
-          flxl = wflux(1, nx, rf, axl, xs, fm1, w2, fp1, xs, nnd);
+          flxl = wflux(1, nx, rf, axl, xs, fm1, w2, fp1, xs, hs);
         } else {
           const T2 xl = X[(pbuf * ncol + col - 1) * VT + lane];
           flxl = xl.x;
           dl = xl.y;
         }
-        const T flxr = wflux(i + 1, nx, rf, axr, fm1, w2, fp1, fp2, xs, nnd);
+        const T flxr = wflux(i + 1, nx, rf, axr, fm1, w2, fp1, fp2, xs, hs);
         const T ayb = vx * T(cc[16]) + vy * T(cc[17]);
         const T ayt = vx * T(cc[18]) + vy * T(cc[19]);
-        const T flyb = first ? wflux(1, ny, cf, ayb, w0, w1, w2, w3, ys, nnd) : fly_prev;
-        const T flyt = wflux(r + 1, ny, cf, ayt, w1, w2, w3, w4, ys, nnd);
+        const T flyb = first ? wflux(1, ny, cf, ayb, w0, w1, w2, w3, ys, hs) : fly_prev;
+        const T flyt = wflux(r + 1, ny, cf, ayt, w1, w2, w3, w4, ys, hs);
         fly_prev = flyt;
         const T rr = -(((flxr - flxl) + flyt) - flyb) * T(cc[9]) + T(cc[8]) * (feq - w2);
         if (store_rhs && act) sto(R, tb + (unsigned)(r + 1) * rb + (unsigned)(i + 1) * cb, rr);
         // forward recursion (src/GKUA_Residual.f90:39-57); Delta*(i,0) = Delta*(0,j) = 0
         const T alfa = T(cc[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
-        const T ra = T(1) / alfa;
+        const T ra = rcp_pos(alfa);
         const T db = first ? T(0) : H[(h1 * 3) * nthr + tl];
         const T ds = (T(cc[10]) * rr + posp(axl) * dl + posp(ayb) * db) * ra;
         T2 xo;
         xo.x = flxr;
         xo.y = ds;
         X[(buf * ncol + col) * VT + lane] = xo;
-        H[(hs * 3 + 0) * nthr + tl] = ds;
-        H[(hs * 3 + 1) * nthr + tl] = -negp(axr) * ra;
-        H[(hs * 3 + 2) * nthr + tl] = -negp(ayt) * ra;
-      }
-      // backward pass + update of position q-2 (src/GKUA_Residual.f90:64-109)
-      if (q >= 2 && q < nseq + 2) {
-        int jf, kf;
-        if (q < nseq) {
-          jf = r - 2;
-          kf = kc;
-          if (jf < -3) {
-            jf += LT;
-            kf = kc - 1;
-          }
-        } else {
-          jf = nrow - 1 + (q - nseq);
-          kf = ktiles - 1;
-        }
-        if (jf >= 1) {
-          const T dr = (i < ncol) ? X[(pbuf * ncol + col + 1) * VT + lane].y : T(0);
-          // Delta*(i, jf+1) is the value of position q-1, 0 past the last row
-          const T dn = (jf == nrow) ? T(0) : H[(h1 * 3) * nthr + tl];
-          const T du =
-              H[(h2 * 3) * nthr + tl] + (H[(h2 * 3 + 1) * nthr + tl] * dr + H[(h2 * 3 + 2) * nthr + tl] * dn);
-          unsigned tb;
-          int ird, vv;
-          if (tile_of(kf, lane, tb, ird, vv)) {
-            T f = w0 + du;
-            if ((double)f < kClamp) f = T(kClamp);
-            sto(F, tb + (unsigned)(jf + 1) * rb + (unsigned)(i + 1) * cb, f);
-          }
-        }
+        H[(hsl * 3 + 0) * nthr + tl] = ds;
+        H[(hsl * 3 + 1) * nthr + tl] = -negp(axr) * ra;
+        H[(hsl * 3 + 2) * nthr + tl] = -negp(ayt) * ra;
       }
     }
     __syncthreads();
@@ -1262,7 +1278,7 @@ bool walk_config(const Dev &d, int tiles, int *vt_out, int *k_out, size_t *lds_o
   const int ntiles = 2 * ((d.nv + vt - 1) / vt);
   const int K = tiles > 0 ? (tiles < ntiles ? tiles : ntiles) : (ntiles / ncu > 1 ? ntiles / ncu : 1);
   const size_t nthr = (size_t)vt * ncol;
-  const size_t lds = 8 * (2 * (size_t)kNW * ncol + 2 * (size_t)K * vt) + 2 * (size_t)ncol * vt * 2 * sizeof(T) +
+  const size_t lds = 8 * (2 * 3 * nthr + 2 * (size_t)K * vt) + 2 * (size_t)ncol * vt * 2 * sizeof(T) +
                      9 * nthr * sizeof(T) + (size_t)std::max(nrow + 2, d.nx + 2);
   if (lds > (size_t)max_lds) return false;
   static bool attr_set = false;
@@ -1287,7 +1303,7 @@ bool launch_walk(const Dev &d, double *cpar, T *F, const T *slot, T *R, int stor
   const size_t slot_bytes = (size_t)8 * d.L * d.nv * sizeof(T);
   if (so < (ptrdiff_t)(2 * (size_t)d.ncell * d.nv * sizeof(T)) || (size_t)so + slot_bytes > 4294967295u) return false;
   const int ncl = (d.nx - 1) * (d.ny - 1);
-  hipLaunchKernelGGL(k_cellpar, dim3((ncl + 255) / 256), dim3(256), 0, s, d, cpar);
+  hipLaunchKernelGGL(k_cellpar, dim3((ncl + 255) / 256), dim3(256), 0, s, d, cpar, 3 * vt * (d.nx - 1));
   const int ntiles = 2 * ((d.nv + vt - 1) / vt);
   const int nb = (ntiles + K - 1) / K;
   const unsigned planeb = (unsigned)((size_t)d.ncell * d.nv * sizeof(T));
