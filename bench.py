@@ -132,6 +132,16 @@ def main():
     bytes_launch = per_unit[names[k]] * (ring_units if names[k] == "moments" else units)
     achieved = bytes_launch / (avg_ms[k] * 1e-3) / 1e9
     step_alg = 4 * esz * units / (el / args.steps) / 1e9
+    # HBM bytes per launch of the dominant kernel measured by rocprofv3
+    # (FETCH_SIZE / WRITE_SIZE, scaled by the copy-kernel calibration) for this
+    # workload, when a profile of it is committed
+    traffic, traffic_src = None, None
+    tfile = ROOT / "profiles" / f"hbm_traffic_cavity{args.n}_v{args.nv}.json"
+    if tfile.exists() and not args.split:
+        rec = json.loads(tfile.read_text()).get(f"cavity{args.n}_v{args.nv}_kn{args.kn}", {})
+        kern = rec.get(args.precision, {}).get("k_" + names[k])
+        if kern:
+            traffic, traffic_src = kern["traffic_bytes"], f"{tfile.relative_to(ROOT)} (rocprofv3 FETCH_SIZE+WRITE_SIZE)"
     out = {
         "metric": "cell·velocity updates/s + time-to-converge vs plain GSIS at 1e-4 rel L2",
         "value": value,
@@ -150,7 +160,7 @@ def main():
                                f"{args.precision}, from rest",
                    "cells": case.ncell, "velocities": case.nv},
         "roofline": {"bound": "hbm", "kernel": names[k], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms[k],
                      "kernel_ms": dict(zip(names, avg_ms)),
                      "step_algorithmic_GBs": step_alg, "step_frac": step_alg / HBM_PEAK_GBS},
