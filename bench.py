@@ -112,6 +112,15 @@ def main():
     t1 = time.perf_counter()
     ms, nt = s.timing(0)
     res = s.residual()
+    fp32_err = None
+    if args.precision == "fp32":
+        # the same steps on the FP64 path (the one checked against the oracle):
+        # relative L2 error of rho, u, T, sigma_xy and q of the FP32 sweeps
+        ref = solver.GsisSolver(case, precision=pkg.capi.PREC_FP64, kstep_show=args.kstep_show)
+        ref.initialize()
+        ref.step(args.warmup + args.steps)
+        fp32_err = pkg.fields.field_errors(s.macro()["s"], ref.macro()["s"], case.nx, case.ny)
+        ref.close()
     s.close()
     el = t1 - t0
     units = case.ncell * case.nv
@@ -166,6 +175,8 @@ def main():
                      "step_algorithmic_GBs": step_alg, "step_frac": step_alg / HBM_PEAK_GBS},
         "residual": list(map(float, res)),
     }
+    if fp32_err is not None:
+        out["fp32_rel_l2_vs_fp64"] = fp32_err
     if args.cpu_steps > 0:
         ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         v, secs = cpu_baseline(case, args.cpu_steps)

@@ -6,8 +6,9 @@ Submodules:
   cases   -- host-side producers: mesh geometry, velocity grids, constants,
              initial state, readers of the reference's input files
   solver  -- GsisSolver: the driver loop over the C-ABI
-  build   -- hipcc build of libgsis.so for gfx950
+  fields  -- macroscopic fields and their relative L2 differences
+  build   -- hipcc build of libgsis.so for gfx950 (and the flang host side)
 """
-from . import capi, cases  # noqa: F401
+from . import capi, cases, fields  # noqa: F401
 
-__all__ = ["capi", "cases"]
+__all__ = ["capi", "cases", "fields"]

@@ -113,10 +113,27 @@ int validate(const gsis_params *p) {
 
 template <typename T>
 void phy_boundary(gsis_handle *h) {
+  // GKUA_Phy_boundary (src/GKUA_boundary.f90:2-46) applies the BCs in input
+  // order.  Diffuse walls read interior cells and write only their own face
+  // slot, so they commute with every other BC and go in one launch; far-field
+  // and periodic fills (which may share corner ghost cells) keep their order.
   T *FG = (T *)h->FG, *FH = (T *)h->FH, *slot = (T *)h->slot;
+  gsis::WallSet ws{};
   for (const BcDev &b : h->bcd) {
-    if (b.code == GSIS_BC_WALL) gsis::launch_wall<T>(h->d, b, FG, FH, slot, h->s);
-    else if (b.code == GSIS_BC_FARFIELD) gsis::launch_farfield<T>(h->d, b, FG, FH, h->s);
+    if (b.code != GSIS_BC_WALL) continue;
+    const int nn = (b.face == 1 || b.face == 3) ? (b.jend - b.jst) : (b.iend - b.ist);
+    if (nn <= 0) continue;
+    if (ws.n == gsis::kMaxWalls) {
+      gsis::launch_walls<T>(h->d, ws, FG, FH, slot, h->s);
+      ws = gsis::WallSet{};
+    }
+    ws.b[ws.n] = b;
+    ws.start[ws.n + 1] = ws.start[ws.n] + nn;
+    ws.n++;
+  }
+  gsis::launch_walls<T>(h->d, ws, FG, FH, slot, h->s);
+  for (const BcDev &b : h->bcd) {
+    if (b.code == GSIS_BC_FARFIELD) gsis::launch_farfield<T>(h->d, b, FG, FH, h->s);
     else if (b.code == GSIS_BC_PERIODIC) gsis::launch_periodic<T>(h->d, b, FG, FH, h->s);
   }
 }

@@ -7,6 +7,15 @@ namespace gsis {
 constexpr int kMaxV1 = 512;  // max discrete velocities per direction (LDS tables)
 
 template <typename T> void launch_wall(const Dev &d, const BcDev &b, T *FG, T *FH, T *slot, hipStream_t s);
+// up to kMaxWalls diffuse-wall subfaces launched together; start[k] is the
+// first block of wall k (start[n] = total face nodes)
+constexpr int kMaxWalls = 16;
+struct WallSet {
+  int n;
+  int start[kMaxWalls + 1];
+  BcDev b[kMaxWalls];
+};
+template <typename T> void launch_walls(const Dev &d, const WallSet &ws, T *FG, T *FH, T *slot, hipStream_t s);
 template <typename T> void launch_farfield(const Dev &d, const BcDev &b, T *FG, T *FH, hipStream_t s);
 template <typename T> void launch_periodic(const Dev &d, const BcDev &b, T *FG, T *FH, hipStream_t s);
 template <typename T> void launch_rhs(const Dev &d, const T *FG, const T *FH, const T *slot, T *RG, T *RH, hipStream_t s);

@@ -90,8 +90,7 @@ __global__ void k_save_s0(Dev d) {
 // reference's icheck/jcheck test).
 // ------------------------------------------------------------------------
 template <typename T>
-__global__ void k_wall(Dev d, BcDev b, T *FG, T *FH, T *slot) {
-  const int kk = blockIdx.x;
+__device__ void wall_node(const Dev &d, const BcDev &b, int kk, T *FG, T *FH, T *slot) {
   int i, j, px, py, px1, py1, pos;
   double swnx, swny, st;
   if (b.face == 1) { i = b.ist; j = b.jst + kk; px = 1; py = j; px1 = 2; py1 = j; pos = j + 1; }
@@ -146,6 +145,20 @@ __global__ void k_wall(Dev d, BcDev b, T *FG, T *FH, T *slot) {
       slot[slot_idx(d, b.face, pos, 1, v)] = T(tw * g / 2.);
     }
   }
+}
+
+template <typename T>
+__global__ void k_wall(Dev d, BcDev b, T *FG, T *FH, T *slot) {
+  wall_node(d, b, blockIdx.x, FG, FH, slot);
+}
+// All diffuse walls of the block in one launch: a wall reads only interior
+// cells and writes only its own face's slot, so the walls are independent of
+// each other and of the far-field / periodic ghost fills.
+template <typename T>
+__global__ void k_walls(Dev d, WallSet ws, T *FG, T *FH, T *slot) {
+  int w = 0;
+  while (w + 1 < ws.n && (int)blockIdx.x >= ws.start[w + 1]) w++;
+  wall_node(d, ws.b[w], (int)blockIdx.x - ws.start[w], FG, FH, slot);
 }
 
 // ------------------------------------------------------------------------
@@ -911,27 +924,40 @@ __global__ void __launch_bounds__(256) k_res4(Dev d, const T *RG, const T *RH) {
 // velocity moments with the DETFP conversion and collision frequency
 // (GaussLegendre src/GKUA_DETFP.f90:288-363, DETFP :52-105).  The reference
 // zeroes all of S_GKUA before the moments (:303), so cells outside
-// [0,nx]x[0,ny] get 0.  One block per padded cell.
+// [0,nx]x[0,ny] get 0.
 // ------------------------------------------------------------------------
+// One wavefront per padded cell.  Lane l covers velocity column iv = l mod nvi
+// of rows jv = l div nvi + R*k (R = 64 div nvi rows per pass when nvi <= 64),
+// or iv = l + 64*m for nvi > 64, so that vi/wi stay in registers and a pass
+// reads consecutive velocities.  Wave-level reduction only (no barrier).
 template <typename T>
 __global__ void __launch_bounds__(256) k_moments(Dev d, T *FG, T *FH) {
-  const int c = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (c >= d.ncell) return;  // whole wavefronts: the reduction below stays wave-uniform
   const int i = c % d.NI - 1, j = c / d.NI - 1;
   const bool interior = (i >= 1 && i <= d.nx - 1 && j >= 1 && j <= d.ny - 1);
   const bool ring = (i >= 0 && i <= d.nx && j >= 0 && j <= d.ny);
   const size_t o = (size_t)c * d.nv;
+  const int nvi = d.nvi, nvj = d.nvj;
+  const int R = nvi <= 64 ? 64 / nvi : 1;  // rows per pass
+  const int nch = nvi <= 64 ? 1 : (nvi + 63) / 64;
   double acc[10];
 #pragma unroll
   for (int k = 0; k < 10; k++) acc[k] = 0.0;
-  for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
-    T g = FG[o + v], h = FH[o + v];
-    if (!interior) {
-      if ((double)g < kClamp) { g = T(kClamp); FG[o + v] = g; }
-      if ((double)h < kClamp) { h = T(kClamp); FH[o + v] = h; }
-    }
-    if (ring) {
-      const int iv = v % d.nvi, jv = v / d.nvi;
-      const double vx = d.vi[iv], vy = d.vj[jv], w = d.wi[iv] * d.wj[jv];
+  for (int ch = 0; ch < nch; ch++) {
+    const int iv = nvi <= 64 ? lane % nvi : ch * 64 + lane;
+    const int jr = nvi <= 64 ? lane / nvi : 0;
+    if (iv >= nvi || jr >= R) continue;
+    const double vx = d.vi[iv], wx = d.wi[iv];
+    for (int jv = jr; jv < nvj; jv += R) {
+      const size_t ov = o + (size_t)jv * nvi + iv;
+      T g = FG[ov], h = FH[ov];
+      if (!interior) {
+        if ((double)g < kClamp) { g = T(kClamp); FG[ov] = g; }
+        if ((double)h < kClamp) { h = T(kClamp); FH[ov] = h; }
+      }
+      const double vy = d.vj[jv], w = wx * d.wj[jv];
       const double gd = (double)g, hd = (double)h;
       const double vx1 = vx * gd, vy1 = vy * gd, vx2 = vx * vx1, vy2 = vy * vy1;
       const double f4 = hd + vx2 + vy2;
@@ -948,11 +974,13 @@ __global__ void __launch_bounds__(256) k_moments(Dev d, T *FG, T *FH) {
     }
   }
   if (!ring) {
-    if (threadIdx.x < 11) d.S[threadIdx.x * d.ncell + c] = 0.0;
-    return;  // uniform per block: no barrier below is skipped by part of it
+    if (lane < 11) d.S[lane * d.ncell + c] = 0.0;
+    return;
   }
-  block_sum<10>(acc);
-  if (threadIdx.x == 0) {
+#pragma unroll
+  for (int k = 0; k < 10; k++)
+    for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off, 64);
+  if (lane == 0) {
     double ron = acc[0];
     if (ron <= 0.) ron = 1.0;  // R00
     const double u = acc[1] / ron, v = acc[2] / ron;
@@ -1222,6 +1250,11 @@ void launch_wall(const Dev &d, const BcDev &b, T *FG, T *FH, T *slot, hipStream_
   hipLaunchKernelGGL(k_wall<T>, dim3(nn), dim3(256), 0, s, d, b, FG, FH, slot);
 }
 template <typename T>
+void launch_walls(const Dev &d, const WallSet &ws, T *FG, T *FH, T *slot, hipStream_t s) {
+  if (ws.n <= 0 || ws.start[ws.n] <= 0) return;
+  hipLaunchKernelGGL(k_walls<T>, dim3(ws.start[ws.n]), dim3(256), 0, s, d, ws, FG, FH, slot);
+}
+template <typename T>
 void launch_farfield(const Dev &d, const BcDev &b, T *FG, T *FH, hipStream_t s) {
   const int nn = (b.face == 1 || b.face == 3) ? (b.jend - b.jst + 1) : (b.iend - b.ist + 1);
   hipLaunchKernelGGL(k_farfield<T>, dim3(2 * nn), dim3(256), 0, s, d, b, FG, FH);
@@ -1314,7 +1347,7 @@ bool launch_walk(const Dev &d, double *cpar, T *F, const T *slot, T *R, int stor
 
 template <typename T>
 void launch_moments(const Dev &d, T *FG, T *FH, hipStream_t s) {
-  hipLaunchKernelGGL(k_moments<T>, dim3(d.ncell), dim3(256), 0, s, d, FG, FH);
+  hipLaunchKernelGGL(k_moments<T>, dim3((d.ncell + 3) / 4), dim3(256), 0, s, d, FG, FH);
 }
 template <typename T>
 void launch_frd_in(const Dev &d, const double *frd, T *FG, T *FH, T *slot, hipStream_t s) {
@@ -1351,6 +1384,7 @@ void launch_residual(const Dev &d, double *out, hipStream_t s) {
 
 #define INST(T)                                                                                   \
   template void launch_wall<T>(const Dev &, const BcDev &, T *, T *, T *, hipStream_t);           \
+  template void launch_walls<T>(const Dev &, const WallSet &, T *, T *, T *, hipStream_t);        \
   template void launch_farfield<T>(const Dev &, const BcDev &, T *, T *, hipStream_t);            \
   template void launch_periodic<T>(const Dev &, const BcDev &, T *, T *, hipStream_t);            \
   template void launch_rhs<T>(const Dev &, const T *, const T *, const T *, T *, T *, hipStream_t); \

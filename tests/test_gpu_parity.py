@@ -152,3 +152,20 @@ def test_residual_cadence_matches_oracle():
     assert sg == sc
     for (ka, ra), (kb, rb) in zip(hg, hc):
         assert ka == kb and rel_l2(ra, rb) <= 1e-9
+
+
+def test_fp32_sweeps_against_fp64():
+    """FP32 storage and sweeps, FP64 moments: the relative L2 error of rho, u,
+    T, sigma_xy and q against the FP64 path after 40 steps of a cavity is
+    reported and stays within the FP32 bounds below."""
+    g = load_pkg()
+    case = g.cases.cavity_case(32, 32, kn=0.1, lid=0.15, dt=1.0)
+    f64 = solver_mod().GsisSolver(case)
+    f32 = solver_mod().GsisSolver(case, precision=g.capi.PREC_FP32)
+    for s in (f64, f32):
+        s.initialize()
+        s.step(40)
+    err = g.fields.field_errors(f32.macro()["s"], f64.macro()["s"], case.nx, case.ny)
+    print("fp32 vs fp64 rel L2:", {k: f"{v:.2e}" for k, v in err.items()}, flush=True)
+    bound = {"rho": 1e-6, "u": 1e-4, "T": 1e-6, "sigma_xy": 1e-3, "q": 1e-3}
+    assert all(err[k] <= bound[k] for k in bound), err
