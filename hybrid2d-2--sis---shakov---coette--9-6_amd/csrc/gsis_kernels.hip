@@ -600,19 +600,19 @@ __device__ __forceinline__ void sto(T *base, unsigned off, T v) {
 // of them for VT consecutive velocities; F holds the G plane followed by the
 // H plane.  Thread (column i, lane) of a block owns mesh column i and one
 // velocity of the tile.  A block walks K tiles back to back: at step t column
-// i is at position q = t - (i-1) of its sequence (tile q / (ny-1), row
-// q mod (ny-1) + 1), so all active columns sit on the same wrapped
-// anti-diagonal and the forward recursion's inputs from column i-1 (its
-// face flux and Delta*) were produced one step earlier and come through LDS,
-// while the j-stencil F(i, j-2..j+2) slides in registers.  Column i finalises
-// position q-2 at step t (backward pass + update): it needs Delta*(i+1, .),
-// published by column i+1 at step t-1, and its own Delta* of the next row,
-// kept with the backward coefficients in a 3-deep LDS ring.  F is updated in
-// place: every other reader of a value has consumed it at an earlier step
-// (column i+1 reads it at t+1 and waits for it before the barrier that ends
-// t+1; the write is at t+2).  All global loads of a position are issued one
-// step ahead, and the velocity-independent constants of the next step's
-// cells are staged cooperatively into LDS one step ahead.
+// i is at position q = t - (i-1) of its sequence (a tile is ny+3 positions:
+// three window-filling virtual rows, then rows 1..ny-1), so all active
+// columns sit on the same wrapped anti-diagonal and the forward recursion's
+// inputs from column i-1 (its face flux and Delta*) were produced one step
+// earlier and come through LDS, while the j-stencil F(i, j-2..j+2) slides in
+// registers.  Column i finalises position q-2 at the top of step t (backward
+// pass + update): it needs Delta*(i+1, .), published by column i+1 at step
+// t-1, and its own Delta* of the next row, kept in registers with the
+// backward coefficients.  F is updated in place: every other reader of a
+// value has consumed it at an earlier step (column i+1 reads it at t+1 and
+// waits for it at the top of t+2, before the write).  All global loads of a
+// position are issued one step ahead, and the velocity-independent constants
+// of the cells two steps ahead are staged cooperatively into LDS.
 // Offsets are 32-bit byte offsets from F, which holds the G plane, the H
 // plane and the wall slots (slot at byte offset slotb) in one allocation of
 // less than 4 GiB.
@@ -631,8 +631,7 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
   double *C = lds_d;                                            // [2][nconst]: [ncol][kNW] + padding
   double2 *V = reinterpret_cast<double2 *>(C + 2 * nconst);     // [K][VT] velocities of the tiles
   T2 *X = reinterpret_cast<T2 *>(V + K * VT);                   // [2][ncol][VT] (face flux, Delta*)
-  T *H = reinterpret_cast<T *>(X + 2 * ncol * VT);              // [3][3][nthr] history ring
-  uint8_t *OF = reinterpret_cast<uint8_t *>(H + 9 * nthr);      // override flags, rows and columns
+  uint8_t *OF = reinterpret_cast<uint8_t *>(X + 2 * ncol * VT);  // override flags, rows and columns
   const int ntiles = 2 * ((nv + VT - 1) / VT);
   const int tile0 = blockIdx.x * K;
   const int ktiles = min(K, ntiles - tile0);
@@ -689,6 +688,9 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
   }
   T w0 = T(0), w1 = T(0), w2 = T(0), w3 = T(0), w4 = T(0);  // F(i, r-2..r+2)
   T fly_prev = T(0);
+  // forward values of positions q-1 (n) and q-2 (o): Delta* and the
+  // backward-pass coefficients -A^-_x/alfa, -A^-_y/alfa
+  T hd_n = T(0), bx_n = T(0), by_n = T(0), hd_o = T(0), bx_o = T(0), by_o = T(0);
   int r = -4, kc = 0;  // row and tile of the current position
   // The constants of step s are loaded at step s-2 and written to LDS buffer
   // s&1 at the top of step s-1, after that step's wait for the loads issued
@@ -725,7 +727,6 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
     w2 = w3;
     w3 = w4;
     w4 = pw;
-    const int h1 = (t + 2) % 3, h2 = (t + 1) % 3, hsl = t % 3;  // ring slots of q-1, q-2, q
     // Backward pass + update of position q-2 (src/GKUA_Residual.f90:64-109),
     // first thing in the step: everything it reads was written before the
     // last barrier, and its store then has the whole step to complete.
@@ -745,9 +746,8 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
       if (jf >= 1) {
         const T dr = (i < ncol) ? X[(pbuf * ncol + col + 1) * VT + lane].y : T(0);
         // Delta*(i, jf+1) is the value of position q-1, 0 past the last row
-        const T dn = (jf == nrow) ? T(0) : H[(h1 * 3) * nthr + tl];
-        const T du =
-            H[(h2 * 3) * nthr + tl] + (H[(h2 * 3 + 1) * nthr + tl] * dr + H[(h2 * 3 + 2) * nthr + tl] * dn);
+        const T dn = (jf == nrow) ? T(0) : hd_n;
+        const T du = hd_o + (bx_o * dr + by_o * dn);
         unsigned tb;
         int ird, vv;
         if (tile_of(kf, lane, tb, ird, vv)) {
@@ -798,15 +798,13 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
       // forward recursion (src/GKUA_Residual.f90:39-57)
       const T alfa = T(cc[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
       const T ra = rcp_pos(alfa);
-      const T db = H[(h1 * 3) * nthr + tl];
-      const T ds = (T(cc[10]) * rr + posp(axl) * xl.y + posp(ayb) * db) * ra;
+      const T ds = (T(cc[10]) * rr + posp(axl) * xl.y + posp(ayb) * hd_n) * ra;
       T2 xo;
       xo.x = flxr;
       xo.y = ds;
       X[(buf * ncol + col) * VT + lane] = xo;
-      H[(hsl * 3 + 0) * nthr + tl] = ds;
-      H[(hsl * 3 + 1) * nthr + tl] = -negp(axr) * ra;
-      H[(hsl * 3 + 2) * nthr + tl] = -negp(ayt) * ra;
+      hd_o = hd_n; bx_o = bx_n; by_o = by_n;
+      hd_n = ds; bx_n = -negp(axr) * ra; by_n = -negp(ayt) * ra;
       if (store_rhs && act) sto(R, oc, rr);
     } else {
       const T fm1 = pm1, fp1 = pp1, fp2 = pp2, xs = px, ys = py;
@@ -841,6 +839,7 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
         py = ldo(F, ay);
       }
       __builtin_amdgcn_sched_barrier(0);
+      T hd_new = T(0), bx_new = T(0), by_new = T(0);
       if (active && r >= 1) {
         const bool first = r == 1;
         unsigned tb;
@@ -880,16 +879,16 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
         // forward recursion (src/GKUA_Residual.f90:39-57); Delta*(i,0) = Delta*(0,j) = 0
         const T alfa = T(cc[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
         const T ra = rcp_pos(alfa);
-        const T db = first ? T(0) : H[(h1 * 3) * nthr + tl];
+        const T db = first ? T(0) : hd_n;
         const T ds = (T(cc[10]) * rr + posp(axl) * dl + posp(ayb) * db) * ra;
         T2 xo;
         xo.x = flxr;
         xo.y = ds;
         X[(buf * ncol + col) * VT + lane] = xo;
-        H[(hsl * 3 + 0) * nthr + tl] = ds;
-        H[(hsl * 3 + 1) * nthr + tl] = -negp(axr) * ra;
-        H[(hsl * 3 + 2) * nthr + tl] = -negp(ayt) * ra;
+        hd_new = ds; bx_new = -negp(axr) * ra; by_new = -negp(ayt) * ra;
       }
+      hd_o = hd_n; bx_o = bx_n; by_o = by_n;
+      hd_n = hd_new; bx_n = bx_new; by_n = by_new;
     }
     __syncthreads();
   }
@@ -1312,7 +1311,7 @@ bool walk_config(const Dev &d, int tiles, int *vt_out, int *k_out, size_t *lds_o
   const int K = tiles > 0 ? (tiles < ntiles ? tiles : ntiles) : (ntiles / ncu > 1 ? ntiles / ncu : 1);
   const size_t nthr = (size_t)vt * ncol;
   const size_t lds = 8 * (2 * 3 * nthr + 2 * (size_t)K * vt) + 2 * (size_t)ncol * vt * 2 * sizeof(T) +
-                     9 * nthr * sizeof(T) + (size_t)std::max(nrow + 2, d.nx + 2);
+                     (size_t)std::max(nrow + 2, d.nx + 2);
   if (lds > (size_t)max_lds) return false;
   static bool attr_set = false;
   if (lds > 65536 && !attr_set) {
