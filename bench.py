@@ -84,7 +84,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps of the CPU-baseline sample (0: skip)")
     ap.add_argument("--kstep-show", type=int, default=50,
                     help="residual cadence (the reference's Kstep_show; the kinetic RHS is kept every N steps)")
-    ap.add_argument("--split", action="store_true", help="separate RHS and sweep kernels instead of the fused one")
+    ap.add_argument("--kernel", choices=("auto", "split", "tiles", "strip16"), default="auto",
+                    help="kinetic kernels: one-pass column-wavefront kernel (auto: 8 columns per workgroup, "
+                         "strip16: 16), separate RHS and sweep kernels, or the one-pass velocity-tile walker")
+    ap.add_argument("--tiles", type=int, default=0, help="velocity tiles per workgroup of the tile walker (0: auto)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy a device buffer of BYTES bytes 5 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -100,7 +103,10 @@ def main():
     solver = _load("gsis_amd.solver", PKG_DIR / "solver.py")
     case = make_case(pkg, args.n, args.nv, args.kn, args.lid, args.dt)
     prec = pkg.capi.PREC_FP32 if args.precision == "fp32" else pkg.capi.PREC_FP64
-    s = solver.GsisSolver(case, precision=prec, kstep_show=args.kstep_show, split_kernels=int(args.split))
+    ksel = {"auto": pkg.capi.KERNEL_AUTO, "split": pkg.capi.KERNEL_SPLIT, "tiles": pkg.capi.KERNEL_TILES,
+            "strip16": pkg.capi.KERNEL_STRIP16}[args.kernel]
+    s = solver.GsisSolver(case, precision=prec, kstep_show=args.kstep_show, split_kernels=ksel,
+                          walk_tiles=args.tiles)
     s.initialize()
     print(f"# case {case.name}: {case.ncell} cells x {case.nv} velocities, precision {args.precision}", flush=True)
     s.step(args.warmup)
@@ -127,14 +133,15 @@ def main():
     value = units * args.steps / el
     esz = 4 if args.precision == "fp32" else 8
     # algorithmic HBM bytes per launch of each kinetic kernel, per interior
-    # cell-velocity: the fused walker reads and writes G and H once (4*esz);
-    # the split path's RHS kernel reads F and writes the RHS (4*esz) and its
-    # sweep reads RHS + F and writes F (6*esz); moments read F over the
-    # padded ring.  The whole step's algorithmic figure is 4*esz.
-    fused = ms[0] < 0.05 * ms[1]
-    names = ["walk", "moments"] if fused else ["rhs", "sweep", "moments"]
+    # cell-velocity: the one-pass kernels (k_strip, k_walk) read and write G
+    # and H once (4*esz); the split path's RHS kernel reads F and writes the
+    # RHS (4*esz) and its sweep reads RHS + F and writes F (6*esz); moments
+    # read F over the padded ring.  The whole step's algorithmic figure is 4*esz.
+    fused = args.kernel != "split"
+    one = "walk" if args.kernel == "tiles" else "strip"
+    names = [one, "moments"] if fused else ["rhs", "sweep", "moments"]
     tot = [ms[1], ms[2]] if fused else list(ms)
-    per_unit = {"walk": 4 * esz, "rhs": 4 * esz, "sweep": 6 * esz, "moments": 2 * esz}
+    per_unit = {"strip": 4 * esz, "walk": 4 * esz, "rhs": 4 * esz, "sweep": 6 * esz, "moments": 2 * esz}
     avg_ms = [m / max(nt, 1) for m in tot]
     k = max(range(len(names)), key=lambda q: avg_ms[q])
     ring_units = (case.nx + 1) * (case.ny + 1) * case.nv
@@ -146,7 +153,7 @@ def main():
     # workload, when a profile of it is committed
     traffic, traffic_src = None, None
     tfile = ROOT / "profiles" / f"hbm_traffic_cavity{args.n}_v{args.nv}.json"
-    if tfile.exists() and not args.split:
+    if tfile.exists() and fused:
         rec = json.loads(tfile.read_text()).get(f"cavity{args.n}_v{args.nv}_kn{args.kn}", {})
         kern = rec.get(args.precision, {}).get("k_" + names[k])
         if kern:

@@ -33,6 +33,11 @@ MODE_M1 = 1
 
 PREC_FP64 = 0
 PREC_FP32 = 1
+# kinetic kernels (gsis_params.split_kernels)
+KERNEL_AUTO = 0      # one-pass column-wavefront kernel, 8 columns per workgroup
+KERNEL_SPLIT = 1     # separate RHS and LU-SGS sweep kernels
+KERNEL_TILES = 2     # one-pass velocity-tile walker
+KERNEL_STRIP16 = 3   # column-wavefront kernel, 16 columns per workgroup
 
 c_dp = ctypes.POINTER(ctypes.c_double)
 

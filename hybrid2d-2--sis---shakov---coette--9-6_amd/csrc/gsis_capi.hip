@@ -6,7 +6,9 @@
 //   DETFP + DETFPwall, Convert_NS (U, HoT), [synthetic NS], counters.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -45,6 +47,7 @@ struct gsis_handle {
   void *FG = nullptr, *FH = nullptr, *slot = nullptr, *RG = nullptr, *RH = nullptr;
   std::vector<void *> allocs;
   double *cpar = nullptr;  // per-cell constants of the fused kinetic kernel
+  gsis::StripBufs strip;   // seam buffers of the column-wavefront kernel
   int *wall_list = nullptr;
   int nwall = 0;
   double *swall = nullptr;
@@ -95,6 +98,8 @@ int validate(const gsis_params *p) {
   if (p->mode != GSIS_MODE_M0 && p->mode != GSIS_MODE_M1) return fail(GSIS_EARG, "unknown mode");
   if (p->dt <= 0.0) return fail(GSIS_EARG, "dt must be positive");
   if (p->walk_tiles < 0 || p->kstep_show < 0) return fail(GSIS_EARG, "walk_tiles and kstep_show must be >= 0");
+  if (p->split_kernels < GSIS_KERNEL_AUTO || p->split_kernels > GSIS_KERNEL_STRIP16)
+    return fail(GSIS_EARG, "split_kernels must be 0 (automatic), 1 (split), 2 (tile walker) or 3 (16-column strips)");
   for (int b = 0; b < p->nbc; b++) {
     const gsis_bc &c = p->bc[b];
     bool ok = false;
@@ -165,10 +170,23 @@ void kinetic(gsis_handle *h) {
   const int store = (show <= 0 || (h->kstep + 1) % show == 0) ? 1 : 0;
   T *FG = (T *)h->FG, *FH = (T *)h->FH, *RG = (T *)h->RG, *RH = (T *)h->RH;
   const T *slot = (const T *)h->slot;
-  if (!h->p.split_kernels) {
-    if (h->timing) { (void)hipEventRecord(h->ev[1], h->s); (void)hipEventRecord(h->ev[2], h->s); }
-    if (gsis::launch_walk<T>(h->d, h->cpar, FG, slot, RG, store, h->p.walk_tiles, h->s)) {
-      if (h->timing) (void)hipEventRecord(h->ev[3], h->s);
+  const int sel = h->p.split_kernels;
+  if (sel != GSIS_KERNEL_SPLIT) {
+    // timing: ev[2]..ev[3] bracket the one-pass kernel itself
+    if (h->timing) (void)hipEventRecord(h->ev[1], h->s);
+    bool ok = false;
+    if (sel == GSIS_KERNEL_AUTO || sel == GSIS_KERNEL_STRIP16) {
+      h->strip.ev_before = h->timing ? h->ev[2] : nullptr;
+      h->strip.ev_after = h->timing ? h->ev[3] : nullptr;
+      ok = gsis::launch_strip<T>(h->d, h->cpar, FG, slot, RG, store, sel == GSIS_KERNEL_STRIP16 ? 16 : 8, h->strip,
+                                 h->s);
+    }
+    if (!ok) {
+      if (h->timing) (void)hipEventRecord(h->ev[2], h->s);
+      ok = gsis::launch_walk<T>(h->d, h->cpar, FG, slot, RG, store, h->p.walk_tiles, h->s);
+      if (ok && h->timing) (void)hipEventRecord(h->ev[3], h->s);
+    }
+    if (ok) {
       if (store) gsis::launch_res4<T>(h->d, RG, RH, h->s);
       return;
     }
@@ -199,10 +217,11 @@ int one_step(gsis_handle *h) {
 }
 
 int check_err_flag(gsis_handle *h) {
-  int e = 0;
-  HIPCHK(hipMemcpyAsync(&e, h->d.err, sizeof(int), hipMemcpyDeviceToHost, h->s));
+  int e[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(e, h->d.err, sizeof(e), hipMemcpyDeviceToHost, h->s));
   HIPCHK(hipStreamSynchronize(h->s));
-  if (e & 1) return fail(GSIS_ETEMP, "T < 0.1 in an interior cell (comput_duvtpckw stop)");
+  if (e[0] & 1) return fail(GSIS_ETEMP, "T < 0.1 in an interior cell (comput_duvtpckw stop)");
+  if (e[1]) return fail(GSIS_EHIP, "kinetic kernel: a wait for the neighbouring strip timed out");
   if (h->p.mode == GSIS_MODE_M1) {
     int it = 0, bad = 0;
     int rc = gsis::synthetic_read_iters(h->syn, h->s, &it, &bad);
@@ -300,7 +319,9 @@ int gsis_create(const gsis_params *p, gsis_handle **out) {
       (rc = dalloc(h, &d.taoi, 8 * 4 * N)) || (rc = dalloc(h, &d.taoj, 8 * 4 * N)) ||
       (rc = dalloc(h, &d.qi, 8 * 2 * N)) || (rc = dalloc(h, &d.qj, 8 * 2 * N)) ||
       (rc = dalloc(h, &d.prim, 8 * 5 * N)) || (rc = dalloc(h, &d.res4, 8 * 4 * N)) || (rc = dalloc(h, &d.err, 64)) ||
-      (rc = dalloc(h, &h->cpar, 8 * (size_t)(ny + 3) * 3 * 1024)) ||  // ny+3 planes of <= 3*1024 constants
+      // ny+3 planes of <= 3*1024 constants (k_walk), or kNCellPar per interior cell (k_strip)
+      (rc = dalloc(h, &h->cpar, 8 * std::max((size_t)(ny + 3) * 3 * 1024,
+                                             (size_t)(nx - 1) * (ny - 1) * gsis::kNCellPar))) ||
       (rc = dalloc(h, &h->resbuf, 8 * 8)))
     return cleanup(rc);
   {  // Creat_Mesh sets U, U0, U2 to (1,0,0,1) (src/sub_init.f90:159-162)
@@ -354,6 +375,16 @@ int gsis_create(const gsis_params *p, gsis_handle **out) {
       (!wl.empty() && hipMemcpy(h->wall_list, wl.data(), 4 * wl.size(), hipMemcpyHostToDevice) != hipSuccess))
     return cleanup(fail(GSIS_EHIP, "BC table upload failed"));
   d.ovrI1 = o1; d.ovrI3 = o3; d.ovrJ2 = o2; d.ovrJ4 = o4;
+  {  // the same flags as 32-bit words for the column-wavefront kernel's scalar loads
+    std::vector<int> of(ny + 2 + nx + 2, 0);
+    for (int j = 0; j < ny + 2; j++) of[j] = (oi1[j] ? 1 : 0) | (oi3[j] ? 2 : 0);
+    for (int i = 0; i < nx + 2; i++) of[ny + 2 + i] = (oj2[i] ? 1 : 0) | (oj4[i] ? 2 : 0);
+    int *ofd;
+    if ((rc = dalloc(h, &ofd, sizeof(int) * of.size()))) return cleanup(rc);
+    if (hipMemcpy(ofd, of.data(), sizeof(int) * of.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(GSIS_EHIP, "BC table upload failed"));
+    h->strip.oflags = ofd;
+  }
   // distributions
   const size_t nf = (size_t)d.ncell * d.nv * h->tsize;
   const size_t ns = (size_t)4 * d.L * 2 * d.nv * h->tsize;
@@ -363,6 +394,19 @@ int gsis_create(const gsis_params *p, gsis_handle **out) {
   h->FH = (char *)h->FG + nf;
   h->slot = (char *)h->FG + 2 * nf;
   h->RH = (char *)h->RG + nf;
+  {  // seam buffers of the column-wavefront kernel, sized for 8 columns per workgroup
+    const size_t rows = gsis::strip_rows(d, 8);
+    gsis::StripBufs &sb = h->strip;
+    if (rows > 0) {
+      if ((rc = dalloc(h, &sb.msg, rows * 256 * h->tsize)) || (rc = dalloc(h, &sb.dright, rows * 128 * h->tsize)) ||
+          (rc = dalloc(h, &sb.flag, rows * sizeof(int))))
+        return cleanup(rc);
+      sb.msg_elems = rows * 256;
+      sb.dright_elems = rows * 128;
+      sb.flag_elems = rows;
+    }
+    sb.err = d.err + 1;
+  }
   if (p->mode == GSIS_MODE_M1) {
     rc = gsis::synthetic_init(h->d, *p, h->syn);
     if (rc) return cleanup(fail(rc, gsis::synthetic_error()));

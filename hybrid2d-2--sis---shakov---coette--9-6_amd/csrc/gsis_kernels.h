@@ -25,6 +25,26 @@ constexpr int kNCellPar = 20;  // per-cell constants of the fused kinetic kernel
 // Fused kinetic step over F = [G plane | H plane] (and R likewise); false when
 // the mesh does not fit the fused kernel.
 template <typename T> bool launch_walk(const Dev &d, double *cpar, T *F, const T *slot, T *R, int store_rhs, int tiles, hipStream_t s);
+// Buffers of the column-wavefront kernel (k_strip): seam messages
+// [seam][tile][row][4][64] and the next strip's Delta* [seam][tile][row][2][64]
+// (elements of the working precision), per-row ready flags holding the
+// launch epoch, and a word set when a wait for a flag timed out.
+struct StripBufs {
+  void *msg = nullptr, *dright = nullptr;
+  int *flag = nullptr, *err = nullptr;
+  // face-override words: ny+2 rows (bit 0 i-face 1, bit 1 i-face nx), then
+  // nx+2 columns (bit 0 j-face 1, bit 1 j-face ny)
+  const int *oflags = nullptr;
+  size_t msg_elems = 0, dright_elems = 0, flag_elems = 0;
+  int epoch = 0;
+  // when set, recorded on the stream just before and after k_strip itself
+  hipEvent_t ev_before = nullptr, ev_after = nullptr;
+};
+// seam rows (seam x tile x row) of a mesh with nw columns per workgroup
+size_t strip_rows(const Dev &d, int nw);
+// Column-wavefront kinetic step with nw (8 or 16) columns per workgroup, plus
+// the seam update; false when the mesh or the buffers do not fit.
+template <typename T> bool launch_strip(const Dev &d, double *cpar, T *F, const T *slot, T *R, int store_rhs, int nw, StripBufs &b, hipStream_t s);
 template <typename T> void launch_res4(const Dev &d, const T *RG, const T *RH, hipStream_t s);
 template <typename T> void launch_frd_in(const Dev &d, const double *frd, T *FG, T *FH, T *slot, hipStream_t s);
 template <typename T> void launch_frd_out(const Dev &d, double *frd, const T *FG, const T *FH, const T *slot, hipStream_t s);

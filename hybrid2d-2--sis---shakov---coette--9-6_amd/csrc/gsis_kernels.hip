@@ -486,6 +486,8 @@ __global__ void __launch_bounds__(256) k_sweep(Dev d, int VT, T *FG, T *FH, cons
 // 12-13 i-face i, 14-15 i-face i+1, 16-17 j-face j, 18-19 j-face j+1.
 // ------------------------------------------------------------------------
 constexpr int kNW = kNCellPar;
+// With pstride = 0 the constants are stored by cell instead, cpar[((j-1)*(nx-1) + i-1)*kNW + k]
+// (the layout of the column-wavefront kernel).
 __global__ void k_cellpar(Dev d, double *cpar, int pstride) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ncol = d.nx - 1, nrow = d.ny - 1;
@@ -493,7 +495,7 @@ __global__ void k_cellpar(Dev d, double *cpar, int pstride) {
   const int i = t % ncol + 1, j = t / ncol + 1;
   const int c = cidx(d, i, j), N = d.ncell;
   const int r = (i + j + 2) % (nrow + 4);
-  double *o = cpar + (size_t)r * pstride + (size_t)(i - 1) * kNW;
+  double *o = pstride ? cpar + (size_t)r * pstride + (size_t)(i - 1) * kNW : cpar + (size_t)t * kNW;
   const double n = d.S[c], T = d.S[3 * N + c];
   const int g0 = gidx(d, i, j), gi = gidx(d, i + 1, j), gj = gidx(d, i, j + 1);
   const double vol = d.vol[g0], cf = d.CF[c];
@@ -892,6 +894,388 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
     }
     __syncthreads();
   }
+}
+
+// ------------------------------------------------------------------------
+// Column-wavefront kinetic step ("strip walker"): the same single pass as
+// k_walk (Shakhov equilibrium, NND fluxes, forward LU-SGS recursion,
+// non-recursive backward pass, update and clamp; src/GKUA_Residual.f90:26-109,
+// 117-469), mapped so that one wavefront owns one mesh column and 64
+// consecutive velocities, with G and H of a velocity in the same lane.  Row,
+// column, wall rules and the 20 cell constants are then wave-uniform (scalar
+// registers and scalar loads), every access to F is one contiguous line per
+// wavefront, and the work that depends only on the cell and the velocity
+// (exp, Shakhov factor, face coefficients, LU-SGS diagonal) is shared by G
+// and H.
+//
+// A workgroup holds NW consecutive columns (a strip) of one velocity tile.
+// Wavefront w is at position q = s - w of the tile's sequence at step s
+// (three window-filling virtual rows, then rows 1..ny-1), so the forward
+// recursion's inputs from column i-1 were published one step earlier in LDS;
+// column i finalises position q-2 at the top of step s, using Delta*(i+1, .)
+// published by column i+1 one step earlier.
+//
+// The strips of a tile form a pipeline across workgroups.  The last column
+// of strip g hands the face flux and Delta* of each row (G and H) to the
+// first column of strip g+1 through a global seam buffer, then raises that
+// row's flag to the launch's epoch; the consumer checks the flag one step
+// before it loads the row.  The grid is strip-major, so a workgroup only
+// waits for one with a lower index, which was dispatched before it: the
+// waits cannot form a cycle.  The waits are bounded; a timeout sets *werr.
+// The last column of a strip is not updated here: its backward pass needs
+// Delta* of the next strip's first column, which k_seam applies after this
+// kernel (the first column stores it in dright).  Until then the column keeps
+// its old values, which are the ones its neighbours read.
+// ------------------------------------------------------------------------
+constexpr int kSpinMax = 1 << 20;  // about a second of polling
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// element of a wavefront-uniform base plus a per-lane byte offset (a global
+// access with a scalar base address and one vector offset register)
+template <typename T>
+__device__ __forceinline__ T ldv(const T *base, unsigned voff) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + voff);
+}
+template <typename T>
+__device__ __forceinline__ void stv(T *base, unsigned voff, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + voff) = v;
+}
+
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// waits for every outstanding vector memory operation (its fence covers the
+// LDS writes of global_load_lds); here the staged copies are read only by the
+// wavefront that issued them, after its own wait, so they stay in flight.
+__device__ __forceinline__ void barrier_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// 16 bytes per lane from global memory straight into LDS at dst + 16*lane
+// (gfx950 global_load_lds_dwordx4; dst is wavefront-uniform).  The builtin
+// exists only for the device target; the host pass compiles the kernel's
+// launch stub and never this body.
+__device__ __forceinline__ void lds_copy16(const void *src, void *dst) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+#endif
+}
+
+template <typename T, int NW>
+__global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restrict__ cpar, T *__restrict__ F,
+                                                 const T *__restrict__ slot, T *__restrict__ R, int store_rhs,
+                                                 int ntiles, T *msg, T *dright, int *flag, int epoch, int *werr,
+                                                 const int *__restrict__ oflags) {
+  using T2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+  constexpr unsigned ES = sizeof(T);
+  __shared__ T2 XG[2][NW][64], XH[2][NW][64];  // (face flux, Delta*) of each column
+  // Values of the next row, copied from global memory straight into LDS
+  // (global_load_lds, 16 bytes per lane) one step ahead, so that no vector
+  // register holds them while they are in flight: per column, slots
+  // 0 F(i, r+2), 1 F(i-1, r), 2 F(i+1, r), 3 F(i+2, r), 4 x-boundary value,
+  // 5 y-boundary value, each the tile's 64 velocities of G then of H.
+  __shared__ T STG[NW][6][128];
+  constexpr int kPer = 16 / sizeof(T);  // velocities per lane and copy
+  constexpr int kLanes = 64 / kPer;     // lanes per component
+  const int nx = d.nx, ny = d.ny, ncol = nx - 1, nrow = ny - 1, nv = d.nv;
+  const int ngroups = (ncol + NW - 1) / NW;
+  const int LT = nrow + 4;                 // positions: virtual rows -3..0, rows 1..ny-1
+  const int nsteps = (NW - 1) + LT + 2;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = (int)blockIdx.x / ntiles, tile = (int)blockIdx.x - g * ntiles;
+  const int i = g * NW + w + 1;  // mesh column of this wavefront
+  if (i > ncol) {  // past the last column: keep the workgroup's barriers
+    for (int s = 0; s < nsteps; s++) barrier_lds();
+    return;
+  }
+  T vx, vy;
+  {
+    const int vv = min(tile * 64 + (int)(threadIdx.x & 63), nv - 1);
+    vx = T(d.vi[vv % d.nvi]);
+    vy = T(d.vj[vv / d.nvi]);
+  }
+  const T hs = d.scheme != 0 ? T(0.5) : T(0);
+  const int cfl = oflags[ny + 2 + i];                     // j-face overrides of column i
+  const bool consumer = w == 0 && g > 0;                  // reads the seam message of strip g-1
+  const bool producer = w == NW - 1 && g < ngroups - 1;   // writes it; its column is updated by k_seam
+  const bool xlo = i == 1, xhi = i == ncol;
+  const bool xrule = i == 1 || i >= ncol - 1;  // an overridden i-face within reach of the right face
+  // byte strides: one cell's velocities, one row of cells, one plane
+  const long long cb = (long long)nv * ES, rb = (long long)d.NI * cb, pb = (long long)d.ncell * cb;
+  char *Fb = reinterpret_cast<char *>(F);
+  char *Rb = reinterpret_cast<char *>(R);
+  const char *Sb = reinterpret_cast<const char *>(slot);
+  // Per-lane offsets are re-derived where they are used, from a copy of the
+  // thread index the compiler cannot see through: keeping them live across
+  // the loop costs more vector registers than the few instructions that
+  // recompute them.
+  // Staged copies: lane l < kLanes copies velocities tile*64 + l*kPer .. +kPer-1
+  // of G, the next kLanes lanes the same of H (one plane, or one slot row,
+  // further).  The last copy of a partial tile is moved back into the
+  // velocity range (those lanes' velocities are not used).
+  auto lane_of = [&]() {
+    unsigned t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return (int)(t & 63);
+  };
+  auto copyF = [&](int k, long long off) {
+    const int l = lane_of(), comp = l / kLanes;
+    const unsigned pvb = (unsigned)min(tile * 64 + (l % kLanes) * kPer, nv - kPer) * ES;
+    if (comp < 2) lds_copy16(Fb + off + (comp ? pb : 0ll) + pvb, &STG[w][k][0]);
+  };
+  auto copyS = [&](int k, int face, int pos) {  // wall slot row of (face, pos), G then H
+    const long long off = (((long long)(face - 1) * d.L + pos) * 2) * cb;
+    const int l = lane_of(), comp = l / kLanes;
+    const unsigned pvb = (unsigned)min(tile * 64 + (l % kLanes) * kPer, nv - kPer) * ES;
+    if (comp < 2) lds_copy16(Sb + off + (comp ? cb : 0ll) + pvb, &STG[w][k][0]);
+  };
+  T mfg = T(0), mdg = T(0), mfh = T(0), mdh = T(0);  // seam message of row r+1
+  int mflag = 0;                                     // flag of row r+2
+  long long oc = -2 * rb + (long long)(i + 1) * cb;  // byte offset of cell (i, r), r = -3
+  long long ocp = ((long long)(-4) * ncol + (i - 1)) * (kNW * 8);  // cpar byte offset of cell (i, r)
+  const long long ccstep = (long long)ncol * (kNW * 8);
+  const size_t sbc = ((size_t)(g > 0 ? g - 1 : 0) * ntiles + tile) * (size_t)nrow;  // seam rows read (row r at sbc + r - 1)
+  const size_t sbp = ((size_t)g * ntiles + tile) * (size_t)nrow;                    // seam rows written
+  copyF(0, oc + 2 * rb);  // F(i, -1), the first window value
+  T g0 = T(0), g1 = T(0), g2 = T(0), g3 = T(0), g4 = T(0);  // G(i, r-2..r+2)
+  T h0 = T(0), h1 = T(0), h2 = T(0), h3 = T(0), h4 = T(0);
+  T gfly = T(0), hfly = T(0);  // flux through the bottom face of the next row
+  T gd_n = T(0), hd_n = T(0), bx_n = T(0), by_n = T(0), gd_o = T(0), hd_o = T(0), bx_o = T(0), by_o = T(0);
+  int flag_due = 0;  // a seam row written in the previous step waits for its flag
+  for (int s = 0; s < nsteps; s++) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the previous step's copies, loads and stores are complete
+    const int lane = lane_of();
+    const unsigned voff = (unsigned)min(tile * 64 + lane, nv - 1) * ES, loff = (unsigned)lane * ES;
+    const bool act = tile * 64 + lane < nv;
+    const int q = s - w, r = q - 3;
+    const int buf = s & 1, pbuf = buf ^ 1;
+    if (producer && flag_due) {  // its message stores are complete: publish the row
+      if (lane == 0) st_agent(flag + sbp + (r - 2), epoch);
+      flag_due = 0;
+    }
+    if (q >= 0 && q < LT + 2) {
+      const bool real = r >= 1 && r <= nrow;
+      const int rfl = real ? oflags[r] : 0;  // i-face overrides of row r
+      // the values staged in the previous step for this row
+      g0 = g1; g1 = g2; g2 = g3; g3 = g4;
+      h0 = h1; h1 = h2; h2 = h3; h3 = h4;
+      if (r <= nrow) {
+        g4 = STG[w][0][lane];
+        h4 = STG[w][0][64 + lane];
+      }
+      T cgm1 = T(0), cgp1 = T(0), cgp2 = T(0), chm1 = T(0), chp1 = T(0), chp2 = T(0);
+      T cgxs = T(0), chxs = T(0), cgys = T(0), chys = T(0);
+      if (real) {
+        cgm1 = STG[w][1][lane]; chm1 = STG[w][1][64 + lane];
+        cgp1 = STG[w][2][lane]; chp1 = STG[w][2][64 + lane];
+        cgp2 = STG[w][3][lane]; chp2 = STG[w][3][64 + lane];
+        if (xlo || (xhi && (rfl & 2))) { cgxs = STG[w][4][lane]; chxs = STG[w][4][64 + lane]; }
+        if ((r == 1 && (cfl & 1)) || (r == nrow && (cfl & 2))) { cgys = STG[w][5][lane]; chys = STG[w][5][64 + lane]; }
+      }
+      // all staged values are in registers before the copies of the next row overwrite them
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+      const T cmfg = mfg, cmdg = mdg, cmfh = mfh, cmdh = mdh;
+      // backward pass + update of row r-2 (src/GKUA_Residual.f90:64-109)
+      const int rf = r - 2;
+      if (rf >= 1 && rf <= nrow && !producer) {
+        T gr = T(0), hr = T(0);
+        if (i < ncol) {
+          gr = XG[pbuf][w + 1][lane].y;
+          hr = XH[pbuf][w + 1][lane].y;
+        }
+        const bool last = rf == nrow;  // Delta*(i, ny) = 0
+        const T gu = gd_o + (bx_o * gr + by_o * (last ? T(0) : gd_n));
+        const T hu = hd_o + (bx_o * hr + by_o * (last ? T(0) : hd_n));
+        T fg = g0 + gu, fh = h0 + hu;
+        if ((double)fg < kClamp) fg = T(kClamp);
+        if ((double)fh < kClamp) fh = T(kClamp);
+        if (act) {
+          stv(reinterpret_cast<T *>(Fb + oc - 2 * rb), voff, fg);
+          stv(reinterpret_cast<T *>(Fb + oc - 2 * rb + pb), voff, fh);
+        }
+      }
+      // copies of the next row rn (one step ahead)
+      const int rn = r + 1;
+      if (rn <= nrow) copyF(0, oc + 3 * rb);
+      if (rn >= 1 && rn <= nrow) {
+        copyF(1, oc + rb - cb);
+        copyF(2, oc + rb + cb);
+        copyF(3, oc + rb + 2 * cb);
+        const int rfn = oflags[rn];
+        if (xlo) {
+          if (rfn & 1) copyS(4, 1, rn + 1);
+          else copyF(4, oc + rb - 2 * cb);  // ghost cell (-1, rn)
+        } else if (xhi && (rfn & 2)) {
+          copyS(4, 3, rn + 1);
+        }
+        if (rn == 1 && (cfl & 1)) copyS(5, 2, i + 1);
+        else if (rn == nrow && (cfl & 2)) copyS(5, 4, i + 1);
+        if (consumer) {
+          // the flag of row rn was loaded one step ago; wait for it if the
+          // producing strip is behind, then load the row's message
+          const size_t sr = sbc + (rn - 1);
+          int fv = __builtin_amdgcn_readfirstlane(mflag);
+          if (fv != epoch && __builtin_amdgcn_readfirstlane(ld_agent(werr)) == 0) {
+            int it = 0;
+            do {
+              __builtin_amdgcn_s_sleep(2);
+              fv = __builtin_amdgcn_readfirstlane(ld_agent(flag + sr));
+            } while (fv != epoch && ++it < kSpinMax);
+            if (fv != epoch && lane == 0) st_agent(werr, 1);
+          }
+          const char *m = reinterpret_cast<const char *>(msg + sr * 256) + loff;
+          mfg = ld_agent(reinterpret_cast<const T *>(m));
+          mdg = ld_agent(reinterpret_cast<const T *>(m + 64 * ES));
+          mfh = ld_agent(reinterpret_cast<const T *>(m + 128 * ES));
+          mdh = ld_agent(reinterpret_cast<const T *>(m + 192 * ES));
+        }
+      }
+      if (consumer && rn + 1 >= 1 && rn + 1 <= nrow) mflag = ld_agent(flag + sbc + rn);
+      // row r: equilibrium, fluxes, RHS, forward recursion (src/GKUA_Residual.f90:39-57,117-469)
+      T gd_new = T(0), hd_new = T(0), bx_new = T(0), by_new = T(0);
+      if (real) {
+        const double *cc = reinterpret_cast<const double *>(reinterpret_cast<const char *>(cpar) + ocp);
+        T feqg, feqh;
+        {  // Shakhov equilibrium of cell (i,r) (Get_Feq, src/GKUA_Residual.f90:420-469)
+          const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
+          const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
+          const T fvx = T(cc[1]) * (viu * T(cc[6]) + vjv * T(cc[7]));
+          const T gd = T(cc[0]) * wexp(-vij2);
+          const T b = fvx * (T(2) * vij2 - T(2));
+          // G: gd (1 + phi (2c^2 - 4)); H: (T gd / 2)(1 + phi (2c^2 - 2))
+          feqg = gd * ((T(1) - T(2) * fvx) + b);
+          feqh = ((T(cc[3]) * gd) * T(0.5)) * (T(1) + b);
+        }
+        const double *cc2 = cc;
+        const T axl = vx * T(cc2[12]) + vy * T(cc2[13]);
+        const T axr = vx * T(cc2[14]) + vy * T(cc2[15]);
+        const T ayb = vx * T(cc2[16]) + vy * T(cc2[17]);
+        const T ayt = vx * T(cc2[18]) + vy * T(cc2[19]);
+        const T alfa = T(cc2[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
+        const T ra = rcp_pos(alfa);
+        const T vinv = T(cc2[9]), cf = T(cc2[8]), vol = T(cc2[10]);
+        const bool first = r == 1;
+        const bool yrule = first || r >= nrow - 1;  // an overridden j-face within reach of the top face
+        // left face: boundary flux, the seam message or column i-1's value
+        T gfl, gdl, hfl, hdl;
+        if (xlo) {
+          gfl = wflux(1, nx, rfl, axl, cgxs, cgm1, g2, cgp1, cgxs, hs);
+          hfl = wflux(1, nx, rfl, axl, chxs, chm1, h2, chp1, chxs, hs);
+          gdl = T(0);
+          hdl = T(0);
+        } else if (w == 0) {
+          gfl = cmfg; gdl = cmdg; hfl = cmfh; hdl = cmdh;
+        } else {
+          const T2 lg = XG[pbuf][w - 1][lane], lh = XH[pbuf][w - 1][lane];
+          gfl = lg.x; gdl = lg.y; hfl = lh.x; hdl = lh.y;
+        }
+        T gfr, hfr, gft, hft, gfb, hfb;
+        if (xrule) {
+          gfr = wflux(i + 1, nx, rfl, axr, cgm1, g2, cgp1, cgp2, cgxs, hs);
+          hfr = wflux(i + 1, nx, rfl, axr, chm1, h2, chp1, chp2, chxs, hs);
+        } else {
+          gfr = iflux(axr, cgm1, g2, cgp1, cgp2, hs);
+          hfr = iflux(axr, chm1, h2, chp1, chp2, hs);
+        }
+        if (yrule) {
+          gft = wflux(r + 1, ny, cfl, ayt, g1, g2, g3, g4, cgys, hs);
+          hft = wflux(r + 1, ny, cfl, ayt, h1, h2, h3, h4, chys, hs);
+        } else {
+          gft = iflux(ayt, g1, g2, g3, g4, hs);
+          hft = iflux(ayt, h1, h2, h3, h4, hs);
+        }
+        if (first) {
+          gfb = wflux(1, ny, cfl, ayb, g0, g1, g2, g3, cgys, hs);
+          hfb = wflux(1, ny, cfl, ayb, h0, h1, h2, h3, chys, hs);
+        } else {
+          gfb = gfly;
+          hfb = hfly;
+        }
+        gfly = gft;
+        hfly = hft;
+        const T grr = -(((gfr - gfl) + gft) - gfb) * vinv + cf * (feqg - g2);
+        const T hrr = -(((hfr - hfl) + hft) - hfb) * vinv + cf * (feqh - h2);
+        if (store_rhs && act) {
+          stv(reinterpret_cast<T *>(Rb + oc), voff, grr);
+          stv(reinterpret_cast<T *>(Rb + oc + pb), voff, hrr);
+        }
+        gd_new = (vol * grr + posp(axl) * gdl + posp(ayb) * (first ? T(0) : gd_n)) * ra;
+        hd_new = (vol * hrr + posp(axl) * hdl + posp(ayb) * (first ? T(0) : hd_n)) * ra;
+        XG[buf][w][lane] = T2{gfr, gd_new};
+        XH[buf][w][lane] = T2{hfr, hd_new};
+        bx_new = -negp(axr) * ra;
+        by_new = -negp(ayt) * ra;
+        if (producer) {
+          char *m = reinterpret_cast<char *>(msg + (sbp + (r - 1)) * 256) + loff;
+          st_agent(reinterpret_cast<T *>(m), gfr);
+          st_agent(reinterpret_cast<T *>(m + 64 * ES), gd_new);
+          st_agent(reinterpret_cast<T *>(m + 128 * ES), hfr);
+          st_agent(reinterpret_cast<T *>(m + 192 * ES), hd_new);
+          flag_due = 1;
+        }
+        if (consumer) {
+          T *m = dright + (sbc + (r - 1)) * 128;
+          stv(m, loff, gd_new);
+          stv(m + 64, loff, hd_new);
+        }
+      }
+      gd_o = gd_n; hd_o = hd_n; bx_o = bx_n; by_o = by_n;
+      gd_n = gd_new; hd_n = hd_new; bx_n = bx_new; by_n = by_new;
+      oc += rb;
+      ocp += ccstep;
+    }
+    barrier_lds();
+  }
+  if (producer && flag_due) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    if ((threadIdx.x & 63) == 0) st_agent(flag + sbp + (nrow - 1), epoch);
+  }
+}
+
+// Backward pass + update of the last column of every strip but the last
+// (src/GKUA_Residual.f90:64-109): Delta*(i, j) and Delta*(i, j+1) from the
+// seam messages, Delta*(i+1, j) from dright, the coefficients
+// -A^-_x/alfa, -A^-_y/alfa recomputed from the cell constants as k_strip
+// computes them.  One thread per (seam, tile, row, velocity lane).
+template <typename T>
+__global__ void __launch_bounds__(256) k_seam(Dev d, const double *__restrict__ cpar, T *F, const T *msg,
+                                              const T *dright, int ntiles, int nw) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)(t & 63);
+  const size_t row_id = t >> 6;  // (seam * ntiles + tile) * nrow + r - 1
+  const int ncol = d.nx - 1, nrow = d.ny - 1, nv = d.nv;
+  const int ngroups = (ncol + nw - 1) / nw;
+  if (row_id >= (size_t)(ngroups - 1) * ntiles * nrow) return;
+  const int r = (int)(row_id % nrow) + 1;
+  const int st = (int)(row_id / nrow);
+  const int sg = st / ntiles, tile = st - sg * ntiles;
+  const int i = (sg + 1) * nw;  // last column of strip sg
+  const int v = tile * 64 + lane;
+  if (v >= nv) return;
+  const size_t nf = (size_t)d.ncell * nv;
+  const T vx = T(d.vi[v % d.nvi]), vy = T(d.vj[v / d.nvi]);
+  const T gds = msg[row_id * 256 + 64 + lane], hds = msg[row_id * 256 + 192 + lane];
+  const T gdn = r < nrow ? msg[(row_id + 1) * 256 + 64 + lane] : T(0);  // Delta*(i, j+1), 0 past the last row
+  const T hdn = r < nrow ? msg[(row_id + 1) * 256 + 192 + lane] : T(0);
+  const T gdr = dright[row_id * 128 + lane], hdr = dright[row_id * 128 + 64 + lane];
+  const double *cc = cpar + ((size_t)(r - 1) * ncol + (i - 1)) * kNW;
+  const T axl = vx * T(cc[12]) + vy * T(cc[13]);
+  const T axr = vx * T(cc[14]) + vy * T(cc[15]);
+  const T ayb = vx * T(cc[16]) + vy * T(cc[17]);
+  const T ayt = vx * T(cc[18]) + vy * T(cc[19]);
+  const T alfa = T(cc[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
+  const T ra = rcp_pos(alfa);
+  const T bx = -negp(axr) * ra, by = -negp(ayt) * ra;
+  const size_t o = ((size_t)(r + 1) * d.NI + (size_t)(i + 1)) * nv + v;
+  T fg = F[o] + (gds + (bx * gdr + by * gdn));
+  T fh = F[nf + o] + (hds + (bx * hdr + by * hdn));
+  if ((double)fg < kClamp) fg = T(kClamp);
+  if ((double)fh < kClamp) fh = T(kClamp);
+  F[o] = fg;
+  F[nf + o] = fh;
 }
 
 // Conserved moments of the stored RHS per interior cell (GaussLegendre_rs,
@@ -1325,6 +1709,42 @@ bool walk_config(const Dev &d, int tiles, int *vt_out, int *k_out, size_t *lds_o
   *lds_out = lds;
   return true;
 }
+// Column-wavefront kernel with NW columns per workgroup, then the seam update.
+template <typename T, int NW>
+bool launch_strip_nw(const Dev &d, double *cpar, T *F, const T *slot, T *R, int store_rhs, StripBufs &b,
+                     hipStream_t s) {
+  const int ncol = d.nx - 1, nrow = d.ny - 1;
+  if (ncol < 2 || nrow < 2) return false;
+  const int ngroups = (ncol + NW - 1) / NW, ntiles = (d.nv + 63) / 64;
+  const size_t rows = (size_t)(ngroups - 1) * ntiles * nrow;
+  if (rows * 256 > b.msg_elems || rows * 128 > b.dright_elems || rows > b.flag_elems) return false;
+  const int ncl = ncol * nrow;
+  hipLaunchKernelGGL(k_cellpar, dim3((ncl + 255) / 256), dim3(256), 0, s, d, cpar, 0);
+  b.epoch += 1;
+  if (b.ev_before) (void)hipEventRecord(b.ev_before, s);
+  hipLaunchKernelGGL((k_strip<T, NW>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cpar, F, slot, R, store_rhs,
+                     ntiles, (T *)b.msg, (T *)b.dright, b.flag, b.epoch, b.err, b.oflags);
+  if (b.ev_after) (void)hipEventRecord(b.ev_after, s);
+  if (ngroups > 1) {
+    const size_t nthr = rows * 64;
+    hipLaunchKernelGGL(k_seam<T>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, d, cpar, F,
+                       (const T *)b.msg, (const T *)b.dright, ntiles, NW);
+  }
+  return true;
+}
+size_t strip_rows(const Dev &d, int nw) {
+  const int ncol = d.nx - 1, nrow = d.ny - 1;
+  if (ncol < 2 || nrow < 2) return 0;
+  return (size_t)((ncol + nw - 1) / nw - 1) * ((d.nv + 63) / 64) * nrow;
+}
+template <typename T>
+bool launch_strip(const Dev &d, double *cpar, T *F, const T *slot, T *R, int store_rhs, int nw, StripBufs &b,
+                  hipStream_t s) {
+  if (nw == 16) return launch_strip_nw<T, 16>(d, cpar, F, slot, R, store_rhs, b, s);
+  if (nw == 8) return launch_strip_nw<T, 8>(d, cpar, F, slot, R, store_rhs, b, s);
+  return false;
+}
+
 template <typename T>
 bool launch_walk(const Dev &d, double *cpar, T *F, const T *slot, T *R, int store_rhs, int tiles, hipStream_t s) {
   int vt = 0, K = 0;
@@ -1390,6 +1810,7 @@ void launch_residual(const Dev &d, double *out, hipStream_t s) {
   template void launch_sweep<T>(const Dev &, T *, T *, const T *, const T *, hipStream_t);        \
   template void launch_moments<T>(const Dev &, T *, T *, hipStream_t);                            \
   template bool launch_walk<T>(const Dev &, double *, T *, const T *, T *, int, int, hipStream_t);      \
+  template bool launch_strip<T>(const Dev &, double *, T *, const T *, T *, int, int, StripBufs &, hipStream_t); \
   template void launch_res4<T>(const Dev &, const T *, const T *, hipStream_t);                  \
   template void launch_frd_in<T>(const Dev &, const double *, T *, T *, T *, hipStream_t);        \
   template void launch_frd_out<T>(const Dev &, double *, const T *, const T *, const T *, hipStream_t);

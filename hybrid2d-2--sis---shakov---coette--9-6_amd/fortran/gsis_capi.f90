@@ -18,6 +18,8 @@ module gsis_capi
   integer(c_int32_t), parameter :: GSIS_SCHEME_UP1 = 0, GSIS_SCHEME_NND = 1
   integer(c_int32_t), parameter :: GSIS_MODE_M0 = 0, GSIS_MODE_M1 = 1
   integer(c_int32_t), parameter :: GSIS_PREC_FP64 = 0, GSIS_PREC_FP32 = 1
+  integer(c_int32_t), parameter :: GSIS_KERNEL_AUTO = 0, GSIS_KERNEL_SPLIT = 1, GSIS_KERNEL_TILES = 2, &
+                                   GSIS_KERNEL_STRIP16 = 3
 
   ! BC_MSG_TYPE (src/COMMONDATA.f90:34-41) as filled by read_bcin (src/sub_init.f90:266-278)
   type, bind(C) :: gsis_bc_t

@@ -53,6 +53,12 @@ extern "C" {
 #define GSIS_PREC_FP64 0
 #define GSIS_PREC_FP32 1 /* FP32 storage + sweeps, FP64 moments/macro/synthetic */
 
+/* kinetic kernels (gsis_params.split_kernels); every choice gives the same results */
+#define GSIS_KERNEL_AUTO 0    /* one-pass column-wavefront kernel, 8 columns per workgroup, when the mesh fits */
+#define GSIS_KERNEL_SPLIT 1   /* separate RHS and LU-SGS sweep kernels */
+#define GSIS_KERNEL_TILES 2   /* one-pass velocity-tile walker (at most 128 cell columns) */
+#define GSIS_KERNEL_STRIP16 3 /* column-wavefront kernel with 16 columns per workgroup */
+
 /* One boundary subface, mirrors BC_MSG_TYPE (src/COMMONDATA.f90:34-41) as
  * filled by read_bcin (src/sub_init.f90:266-278).  Node indices are 1-based. */
 typedef struct gsis_bc {
@@ -97,8 +103,9 @@ typedef struct gsis_params {
      of a step is kept for gsis_residual only when the step count after it is a
      multiple of kstep_show; 0 keeps it after every step */
   int32_t kstep_show;
-  int32_t split_kernels; /* 1: separate RHS / sweep kernels instead of the fused one */
-  int32_t walk_tiles;    /* velocity tiles per workgroup of the fused kernel; 0 = automatic */
+  int32_t split_kernels; /* GSIS_KERNEL_*: which kinetic kernels run (a one-pass kernel that does not
+                            fit the mesh falls back to the next: strips, tile walker, split) */
+  int32_t walk_tiles;    /* velocity tiles per workgroup of the tile walker; 0 = automatic */
 } gsis_params;
 
 typedef struct gsis_handle gsis_handle;

@@ -16,6 +16,35 @@ def test_header_and_library_agree():
         assert hasattr(lib, name), name
 
 
+def test_no_unresolved_library_symbols():
+    """Every symbol of the package's own code that libgsis.so references is
+    defined in it (a kernel whose host launch stub was not emitted shows up
+    here, not only when its first launch fails)."""
+    import shutil
+    import subprocess
+    nm = shutil.which("nm")
+    if nm is None:
+        import pytest
+        pytest.skip("nm not available")
+    g = load_pkg()
+    out = subprocess.run([nm, "-DC", "--undefined-only", str(g.capi.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    missing = [l for l in out.splitlines() if "gsis" in l]
+    assert not missing, missing
+
+
+def test_kernel_selector_checked_without_gpu():
+    g = load_pkg()
+    lib = g.capi.load_library()
+    case = g.cases.cavity_case(8, 8, kn=0.1)
+    pack = g.cases.ParamPack(case)
+    h = ctypes.c_void_p()
+    for bad in (-1, 4):
+        pack.p.split_kernels = bad
+        assert lib.gsis_create(ctypes.byref(pack.p), ctypes.byref(h)) == g.capi.GSIS_EARG
+        assert b"split_kernels" in lib.gsis_last_error()
+
+
 def test_null_params_rejected_without_gpu():
     g = load_pkg()
     lib = g.capi.load_library()

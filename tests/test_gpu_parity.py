@@ -131,11 +131,53 @@ def test_fused_tile_sequences(n, nv, tiles):
     meshes, against the oracle."""
     g = load_pkg()
     case = g.cases.cavity_case(n, nv, kn=0.1, lid=0.2, dt=0.8)
-    gpu, cpu = _pair(case, walk_tiles=tiles)
+    gpu, cpu = _pair(case, split_kernels=load_pkg().capi.KERNEL_TILES, walk_tiles=tiles)
     gpu.step(3)
     cpu.step(3)
     _compare(gpu, cpu, f"{case.name} fused tiles={tiles}")
     assert rel_l2(gpu.residual(), cpu.residual()) <= TOL
+
+
+# Column-wavefront (strip) kernel: meshes whose last strip is partial or a
+# single column, several strips and seams, partial velocity tiles, walls on
+# every side, the periodic Couette channel, both strip widths.
+@pytest.mark.parametrize("kind,n,nv,kernel", [
+    ("cavity", 19, 20, 0), ("cavity", 17, 24, 0), ("cavity", 41, 24, 3), ("cavity", 33, 20, 3),
+    ("cavity", 9, 16, 0), ("couette", 4, 20, 0), ("couette", 20, 16, 0), ("cavity", 67, 28, 0)],
+    ids=["cavity19_v20_s8", "cavity17_v24_s8_single_last", "cavity41_v24_s16", "cavity33_v20_s16_single_last",
+         "cavity9_v16_s8_one_seam", "couette4_v20_s8", "couette20_v16_s8_periodic_seams", "cavity67_v28_s8"])
+def test_strip_kernel_against_oracle(kind, n, nv, kernel):
+    """The column-wavefront kernel (seam messages between workgroups and the
+    seam update) reproduces the oracle step for step."""
+    g = load_pkg()
+    if kind == "cavity":
+        case = g.cases.cavity_case(n, nv, kn=0.1, lid=0.2, dt=0.8)
+    else:
+        case = g.cases.couette_case(ny_cells=32, nx_cells=n, nv=nv, kn=0.1, uwall=0.1, dt=0.5)
+    print("case:", case.name, "kernel", kernel, flush=True)
+    gpu, cpu = _pair(case, split_kernels=kernel)
+    for k in range(2):
+        gpu.step(2)
+        cpu.step(2)
+        _compare(gpu, cpu, f"{case.name} strip kernel={kernel} step {2 * (k + 1)}")
+    assert rel_l2(gpu.residual(), cpu.residual()) <= TOL
+
+
+def test_one_pass_kernels_agree():
+    """Strips of 8 and 16 columns, the tile walker and the split kernels give
+    the same distributions to rounding on a mesh with several seams."""
+    g = load_pkg()
+    case = g.cases.cavity_case(50, 24, kn=0.05, lid=0.2, dt=0.7)
+    print("case:", case.name, flush=True)
+    ks = [g.capi.KERNEL_AUTO, g.capi.KERNEL_STRIP16, g.capi.KERNEL_TILES, g.capi.KERNEL_SPLIT]
+    runs = [solver_mod().GsisSolver(case, split_kernels=k) for k in ks]
+    for s in runs:
+        s.initialize()
+        s.step(6)
+    ref = runs[-1].frd()
+    for k, s in zip(ks, runs):
+        assert rel_l2(s.frd(), ref) <= 1e-12, k
+        s.close()
 
 
 def test_residual_cadence_matches_oracle():
