@@ -954,6 +954,15 @@ __device__ __forceinline__ void stv(T *base, unsigned voff, T v) {
 // wavefront that issued them, after its own wait, so they stay in flight.
 __device__ __forceinline__ void barrier_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// A wavefront-uniform value the compiler must treat as new at this point:
+// everything derived from it is recomputed (a few scalar instructions) rather
+// than hoisted out of the loop and kept in registers.
+template <typename X>
+__device__ __forceinline__ X fresh(X x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
 // 16 bytes per lane from global memory straight into LDS at dst + 16*lane
 // (gfx950 global_load_lds_dwordx4; dst is wavefront-uniform).  The builtin
 // exists only for the device target; the host pass compiles the kernel's
@@ -984,10 +993,10 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
   const int ngroups = (ncol + NW - 1) / NW;
   const int LT = nrow + 4;                 // positions: virtual rows -3..0, rows 1..ny-1
   const int nsteps = (NW - 1) + LT + 2;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int g = (int)blockIdx.x / ntiles, tile = (int)blockIdx.x - g * ntiles;
-  const int i = g * NW + w + 1;  // mesh column of this wavefront
-  if (i > ncol) {  // past the last column: keep the workgroup's barriers
+  const int w0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g0w = (int)blockIdx.x / ntiles, tile = (int)blockIdx.x - g0w * ntiles;
+  const int i0 = g0w * NW + w0 + 1;  // mesh column of this wavefront
+  if (i0 > ncol) {  // past the last column: keep the workgroup's barriers
     for (int s = 0; s < nsteps; s++) barrier_lds();
     return;
   }
@@ -998,16 +1007,17 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
     vy = T(d.vj[vv / d.nvi]);
   }
   const T hs = d.scheme != 0 ? T(0.5) : T(0);
-  const int cfl = oflags[ny + 2 + i];                     // j-face overrides of column i
-  const bool consumer = w == 0 && g > 0;                  // reads the seam message of strip g-1
-  const bool producer = w == NW - 1 && g < ngroups - 1;   // writes it; its column is updated by k_seam
-  const bool xlo = i == 1, xhi = i == ncol;
-  const bool xrule = i == 1 || i >= ncol - 1;  // an overridden i-face within reach of the right face
+  const int cfl0 = oflags[ny + 2 + i0];  // j-face overrides of column i
+  // Roles, re-derived every step: the first column of strip g > 0 reads the
+  // seam message of strip g-1 (consumer); the last column of a strip that is
+  // not the last writes it (producer) and is updated by k_seam; xlo / xhi:
+  // the mesh's first / last column; xrule: an overridden i-face within reach
+  // of the right face.
   // byte strides: one cell's velocities, one row of cells, one plane
   const long long cb = (long long)nv * ES, rb = (long long)d.NI * cb, pb = (long long)d.ncell * cb;
-  char *Fb = reinterpret_cast<char *>(F);
-  char *Rb = reinterpret_cast<char *>(R);
-  const char *Sb = reinterpret_cast<const char *>(slot);
+  char *const Fb0 = reinterpret_cast<char *>(F);
+  char *const Rb0 = reinterpret_cast<char *>(R);
+  const char *const Sb0 = reinterpret_cast<const char *>(slot);
   // Per-lane offsets are re-derived where they are used, from a copy of the
   // thread index the compiler cannot see through: keeping them live across
   // the loop costs more vector registers than the few instructions that
@@ -1021,25 +1031,29 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
     asm volatile("" : "+v"(t));
     return (int)(t & 63);
   };
-  auto copyF = [&](int k, long long off) {
-    const int l = lane_of(), comp = l / kLanes;
-    const unsigned pvb = (unsigned)min(tile * 64 + (l % kLanes) * kPer, nv - kPer) * ES;
-    if (comp < 2) lds_copy16(Fb + off + (comp ? pb : 0ll) + pvb, &STG[w][k][0]);
+  // per-lane byte offsets of a copy: into F (G or H plane) and into the wall slots
+  auto copy_offsets = [&](int l, long long &stF, long long &stS) {
+    const int comp = l / kLanes;
+    const long long pvb = (long long)min(tile * 64 + (l % kLanes) * kPer, nv - kPer) * ES;
+    stF = (comp ? pb : 0ll) + pvb;
+    stS = (comp ? cb : 0ll) + pvb;
   };
-  auto copyS = [&](int k, int face, int pos) {  // wall slot row of (face, pos), G then H
-    const long long off = (((long long)(face - 1) * d.L + pos) * 2) * cb;
-    const int l = lane_of(), comp = l / kLanes;
-    const unsigned pvb = (unsigned)min(tile * 64 + (l % kLanes) * kPer, nv - kPer) * ES;
-    if (comp < 2) lds_copy16(Sb + off + (comp ? cb : 0ll) + pvb, &STG[w][k][0]);
+  auto copy = [&](int w, int k, const char *base, long long st, int l) {
+    if (l < 2 * kLanes) lds_copy16(base + st, &STG[w][k][0]);
   };
   T mfg = T(0), mdg = T(0), mfh = T(0), mdh = T(0);  // seam message of row r+1
   int mflag = 0;                                     // flag of row r+2
-  long long oc = -2 * rb + (long long)(i + 1) * cb;  // byte offset of cell (i, r), r = -3
-  long long ocp = ((long long)(-4) * ncol + (i - 1)) * (kNW * 8);  // cpar byte offset of cell (i, r)
+  long long oc = -2 * rb + (long long)(i0 + 1) * cb;  // byte offset of cell (i, r), r = -3
+  long long ocp = ((long long)(-4) * ncol + (i0 - 1)) * (kNW * 8);  // cpar byte offset of cell (i, r)
   const long long ccstep = (long long)ncol * (kNW * 8);
-  const size_t sbc = ((size_t)(g > 0 ? g - 1 : 0) * ntiles + tile) * (size_t)nrow;  // seam rows read (row r at sbc + r - 1)
-  const size_t sbp = ((size_t)g * ntiles + tile) * (size_t)nrow;                    // seam rows written
-  copyF(0, oc + 2 * rb);  // F(i, -1), the first window value
+  const size_t sbc = ((size_t)(g0w > 0 ? g0w - 1 : 0) * ntiles + tile) * (size_t)nrow;  // seam rows read (row r at sbc + r - 1)
+  const size_t sbp = ((size_t)g0w * ntiles + tile) * (size_t)nrow;                    // seam rows written
+  {
+    const int l = (int)(threadIdx.x & 63);
+    long long stF, stS;
+    copy_offsets(l, stF, stS);
+    copy(w0, 0, Fb0 + oc + 2 * rb, stF, l);  // F(i, -1), the first window value
+  }
   T g0 = T(0), g1 = T(0), g2 = T(0), g3 = T(0), g4 = T(0);  // G(i, r-2..r+2)
   T h0 = T(0), h1 = T(0), h2 = T(0), h3 = T(0), h4 = T(0);
   T gfly = T(0), hfly = T(0);  // flux through the bottom face of the next row
@@ -1047,6 +1061,15 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
   int flag_due = 0;  // a seam row written in the previous step waits for its flag
   for (int s = 0; s < nsteps; s++) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the previous step's copies, loads and stores are complete
+    // per-wavefront values, re-derived every step (see fresh())
+    const int w = fresh(w0), i = fresh(i0), g = fresh(g0w), cfl = fresh(cfl0);
+    char *const Fb = fresh(Fb0);
+    char *const Rb = fresh(Rb0);
+    const char *const Sb = fresh(Sb0);
+    const bool consumer = w == 0 && g > 0;
+    const bool producer = w == NW - 1 && g < ngroups - 1;
+    const bool xlo = i == 1, xhi = i == ncol;
+    const bool xrule = i == 1 || i >= ncol - 1;
     const int lane = lane_of();
     const unsigned voff = (unsigned)min(tile * 64 + lane, nv - 1) * ES, loff = (unsigned)lane * ES;
     const bool act = tile * 64 + lane < nv;
@@ -1060,59 +1083,66 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
       const bool real = r >= 1 && r <= nrow;
       const int rfl = real ? oflags[r] : 0;  // i-face overrides of row r
       // the values staged in the previous step for this row
+      // (slots that were not copied for this row hold older values, which are
+      // read here but never used)
       g0 = g1; g1 = g2; g2 = g3; g3 = g4;
       h0 = h1; h1 = h2; h2 = h3; h3 = h4;
-      if (r <= nrow) {
-        g4 = STG[w][0][lane];
-        h4 = STG[w][0][64 + lane];
-      }
-      T cgm1 = T(0), cgp1 = T(0), cgp2 = T(0), chm1 = T(0), chp1 = T(0), chp2 = T(0);
-      T cgxs = T(0), chxs = T(0), cgys = T(0), chys = T(0);
+      g4 = STG[w][0][lane];
+      h4 = STG[w][0][64 + lane];
+      const T cgm1 = STG[w][1][lane], chm1 = STG[w][1][64 + lane];
+      const T cgp1 = STG[w][2][lane], chp1 = STG[w][2][64 + lane];
+      const T cgp2 = STG[w][3][lane], chp2 = STG[w][3][64 + lane];
+      const T cgxs = STG[w][4][lane], chxs = STG[w][4][64 + lane];
+      const T cgys = STG[w][5][lane], chys = STG[w][5][64 + lane];
+      const T cmfg = mfg, cmdg = mdg, cmfh = mfh, cmdh = mdh;
+      // row r: equilibrium, fluxes, RHS, forward recursion (src/GKUA_Residual.f90:39-57,117-469)
+      T gd_new = T(0), hd_new = T(0), bx_new = T(0), by_new = T(0);
+      T feqg = T(0), feqh = T(0), axl = T(0), axr = T(0), ayb = T(0), ayt = T(0), ra = T(0);
+      T vinv = T(0), cf = T(0), vol = T(0);
       if (real) {
-        cgm1 = STG[w][1][lane]; chm1 = STG[w][1][64 + lane];
-        cgp1 = STG[w][2][lane]; chp1 = STG[w][2][64 + lane];
-        cgp2 = STG[w][3][lane]; chp2 = STG[w][3][64 + lane];
-        if (xlo || (xhi && (rfl & 2))) { cgxs = STG[w][4][lane]; chxs = STG[w][4][64 + lane]; }
-        if ((r == 1 && (cfl & 1)) || (r == nrow && (cfl & 2))) { cgys = STG[w][5][lane]; chys = STG[w][5][64 + lane]; }
+        const double *cc = reinterpret_cast<const double *>(reinterpret_cast<const char *>(cpar) + ocp);
+        {  // Shakhov equilibrium of cell (i,r) (Get_Feq, src/GKUA_Residual.f90:420-469)
+          const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
+          const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
+          const T fvx = T(cc[1]) * (viu * T(cc[6]) + vjv * T(cc[7]));
+          const T gd = T(cc[0]) * wexp(-vij2);
+          const T b = fvx * (T(2) * vij2 - T(2));
+          // G: gd (1 + phi (2c^2 - 4)); H: (T gd / 2)(1 + phi (2c^2 - 2))
+          feqg = gd * ((T(1) - T(2) * fvx) + b);
+          feqh = ((T(cc[3]) * gd) * T(0.5)) * (T(1) + b);
+        }
+        const double *cc2 = cc;
+        axl = vx * T(cc2[12]) + vy * T(cc2[13]);
+        axr = vx * T(cc2[14]) + vy * T(cc2[15]);
+        ayb = vx * T(cc2[16]) + vy * T(cc2[17]);
+        ayt = vx * T(cc2[18]) + vy * T(cc2[19]);
+        const T alfa = T(cc2[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
+        ra = rcp_pos(alfa);
+        vinv = T(cc2[9]);
+        cf = T(cc2[8]);
+        vol = T(cc2[10]);
       }
       // all staged values are in registers before the copies of the next row overwrite them
       __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-      const T cmfg = mfg, cmdg = mdg, cmfh = mfh, cmdh = mdh;
-      // backward pass + update of row r-2 (src/GKUA_Residual.f90:64-109)
-      const int rf = r - 2;
-      if (rf >= 1 && rf <= nrow && !producer) {
-        T gr = T(0), hr = T(0);
-        if (i < ncol) {
-          gr = XG[pbuf][w + 1][lane].y;
-          hr = XH[pbuf][w + 1][lane].y;
-        }
-        const bool last = rf == nrow;  // Delta*(i, ny) = 0
-        const T gu = gd_o + (bx_o * gr + by_o * (last ? T(0) : gd_n));
-        const T hu = hd_o + (bx_o * hr + by_o * (last ? T(0) : hd_n));
-        T fg = g0 + gu, fh = h0 + hu;
-        if ((double)fg < kClamp) fg = T(kClamp);
-        if ((double)fh < kClamp) fh = T(kClamp);
-        if (act) {
-          stv(reinterpret_cast<T *>(Fb + oc - 2 * rb), voff, fg);
-          stv(reinterpret_cast<T *>(Fb + oc - 2 * rb + pb), voff, fh);
-        }
-      }
       // copies of the next row rn (one step ahead)
       const int rn = r + 1;
-      if (rn <= nrow) copyF(0, oc + 3 * rb);
+      long long stF, stS;
+      copy_offsets(lane, stF, stS);
+      auto slotrow = [&](int face, int pos) { return Sb + (((long long)(face - 1) * d.L + pos) * 2) * cb; };
+      if (rn <= nrow) copy(w, 0, Fb + oc + 3 * rb, stF, lane);
       if (rn >= 1 && rn <= nrow) {
-        copyF(1, oc + rb - cb);
-        copyF(2, oc + rb + cb);
-        copyF(3, oc + rb + 2 * cb);
+        copy(w, 1, Fb + oc + rb - cb, stF, lane);
+        copy(w, 2, Fb + oc + rb + cb, stF, lane);
+        copy(w, 3, Fb + oc + rb + 2 * cb, stF, lane);
         const int rfn = oflags[rn];
         if (xlo) {
-          if (rfn & 1) copyS(4, 1, rn + 1);
-          else copyF(4, oc + rb - 2 * cb);  // ghost cell (-1, rn)
+          if (rfn & 1) copy(w, 4, slotrow(1, rn + 1), stS, lane);
+          else copy(w, 4, Fb + oc + rb - 2 * cb, stF, lane);  // ghost cell (-1, rn)
         } else if (xhi && (rfn & 2)) {
-          copyS(4, 3, rn + 1);
+          copy(w, 4, slotrow(3, rn + 1), stS, lane);
         }
-        if (rn == 1 && (cfl & 1)) copyS(5, 2, i + 1);
-        else if (rn == nrow && (cfl & 2)) copyS(5, 4, i + 1);
+        if (rn == 1 && (cfl & 1)) copy(w, 5, slotrow(2, i + 1), stS, lane);
+        else if (rn == nrow && (cfl & 2)) copy(w, 5, slotrow(4, i + 1), stS, lane);
         if (consumer) {
           // the flag of row rn was loaded one step ago; wait for it if the
           // producing strip is behind, then load the row's message
@@ -1134,29 +1164,7 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
         }
       }
       if (consumer && rn + 1 >= 1 && rn + 1 <= nrow) mflag = ld_agent(flag + sbc + rn);
-      // row r: equilibrium, fluxes, RHS, forward recursion (src/GKUA_Residual.f90:39-57,117-469)
-      T gd_new = T(0), hd_new = T(0), bx_new = T(0), by_new = T(0);
       if (real) {
-        const double *cc = reinterpret_cast<const double *>(reinterpret_cast<const char *>(cpar) + ocp);
-        T feqg, feqh;
-        {  // Shakhov equilibrium of cell (i,r) (Get_Feq, src/GKUA_Residual.f90:420-469)
-          const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
-          const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
-          const T fvx = T(cc[1]) * (viu * T(cc[6]) + vjv * T(cc[7]));
-          const T gd = T(cc[0]) * wexp(-vij2);
-          const T b = fvx * (T(2) * vij2 - T(2));
-          // G: gd (1 + phi (2c^2 - 4)); H: (T gd / 2)(1 + phi (2c^2 - 2))
-          feqg = gd * ((T(1) - T(2) * fvx) + b);
-          feqh = ((T(cc[3]) * gd) * T(0.5)) * (T(1) + b);
-        }
-        const double *cc2 = cc;
-        const T axl = vx * T(cc2[12]) + vy * T(cc2[13]);
-        const T axr = vx * T(cc2[14]) + vy * T(cc2[15]);
-        const T ayb = vx * T(cc2[16]) + vy * T(cc2[17]);
-        const T ayt = vx * T(cc2[18]) + vy * T(cc2[19]);
-        const T alfa = T(cc2[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
-        const T ra = rcp_pos(alfa);
-        const T vinv = T(cc2[9]), cf = T(cc2[8]), vol = T(cc2[10]);
         const bool first = r == 1;
         const bool yrule = first || r >= nrow - 1;  // an overridden j-face within reach of the top face
         // left face: boundary flux, the seam message or column i-1's value
@@ -1222,6 +1230,25 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
           stv(m + 64, loff, hd_new);
         }
       }
+      // backward pass + update of row r-2 (src/GKUA_Residual.f90:64-109)
+      const int rf = r - 2;
+      if (rf >= 1 && rf <= nrow && !producer) {
+        T gr = T(0), hr = T(0);
+        if (i < ncol) {
+          gr = XG[pbuf][w + 1][lane].y;
+          hr = XH[pbuf][w + 1][lane].y;
+        }
+        const bool last = rf == nrow;  // Delta*(i, ny) = 0
+        const T gu = gd_o + (bx_o * gr + by_o * (last ? T(0) : gd_n));
+        const T hu = hd_o + (bx_o * hr + by_o * (last ? T(0) : hd_n));
+        T fg = g0 + gu, fh = h0 + hu;
+        if ((double)fg < kClamp) fg = T(kClamp);
+        if ((double)fh < kClamp) fh = T(kClamp);
+        if (act) {
+          stv(reinterpret_cast<T *>(Fb + oc - 2 * rb), voff, fg);
+          stv(reinterpret_cast<T *>(Fb + oc - 2 * rb + pb), voff, fh);
+        }
+      }
       gd_o = gd_n; hd_o = hd_n; bx_o = bx_n; by_o = by_n;
       gd_n = gd_new; hd_n = hd_new; bx_n = bx_new; by_n = by_new;
       oc += rb;
@@ -1229,7 +1256,7 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
     }
     barrier_lds();
   }
-  if (producer && flag_due) {
+  if (w0 == NW - 1 && g0w < ngroups - 1 && flag_due) {
     __builtin_amdgcn_s_waitcnt(0x0F70);
     if ((threadIdx.x & 63) == 0) st_agent(flag + sbp + (nrow - 1), epoch);
   }
