@@ -1059,6 +1059,10 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
   T gfly = T(0), hfly = T(0);  // flux through the bottom face of the next row
   T gd_n = T(0), hd_n = T(0), bx_n = T(0), by_n = T(0), gd_o = T(0), hd_o = T(0), bx_o = T(0), by_o = T(0);
   int flag_due = 0;  // a seam row written in the previous step waits for its flag
+  // The cell constants of the next row are touched one step ahead (one
+  // dword from each 64-byte line of the 160-byte record), so that the
+  // scalar loads of the row's 20 constants hit the scalar cache.
+  int pf0 = 0, pf1 = 0, pf2 = 0, pf3 = 0;
   for (int s = 0; s < nsteps; s++) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the previous step's copies, loads and stores are complete
     // per-wavefront values, re-derived every step (see fresh())
@@ -1080,6 +1084,14 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
       flag_due = 0;
     }
     if (q >= 0 && q < LT + 2) {
+      asm volatile("" ::"s"(pf0), "s"(pf1), "s"(pf2), "s"(pf3));  // the previous step's touches
+      if (r + 1 >= 1 && r + 1 <= nrow) {
+        const int *pc = reinterpret_cast<const int *>(reinterpret_cast<const char *>(cpar) + ocp + ccstep);
+        pf0 = pc[0];
+        pf1 = pc[16];
+        pf2 = pc[32];
+        pf3 = pc[kNW * 2 - 1];
+      }
       const bool real = r >= 1 && r <= nrow;
       const int rfl = real ? oflags[r] : 0;  // i-face overrides of row r
       // the values staged in the previous step for this row
@@ -1360,6 +1372,9 @@ __global__ void __launch_bounds__(256) k_moments(Dev d, T *FG, T *FH) {
     const int jr = nvi <= 64 ? lane / nvi : 0;
     if (iv >= nvi || jr >= R) continue;
     const double vx = d.vi[iv], wx = d.wi[iv];
+    // unrolled so that several rows' loads are in flight per wavefront (the
+    // sums still run over jv in order)
+#pragma unroll 4
     for (int jv = jr; jv < nvj; jv += R) {
       const size_t ov = o + (size_t)jv * nvi + iv;
       T g = FG[ov], h = FH[ov];
