@@ -996,6 +996,11 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
   const int w0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g0w = (int)blockIdx.x / ntiles, tile = (int)blockIdx.x - g0w * ntiles;
   const int i0 = g0w * NW + w0 + 1;  // mesh column of this wavefront
+  // Strips early in the pipeline feed all later ones: their wavefronts get
+  // the higher issue priority on a CU they share with later strips.
+  if (g0w == 0) __builtin_amdgcn_s_setprio(3);
+  else if (g0w == 1) __builtin_amdgcn_s_setprio(2);
+  else if (g0w == 2) __builtin_amdgcn_s_setprio(1);
   if (i0 > ncol) {  // past the last column: keep the workgroup's barriers
     for (int s = 0; s < nsteps; s++) barrier_lds();
     return;
