@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--kernel", choices=("auto", "split", "tiles", "strip16"), default="auto",
                     help="kinetic kernels: one-pass column-wavefront kernel (auto: 8 columns per workgroup, "
                          "strip16: 16), separate RHS and sweep kernels, or the one-pass velocity-tile walker")
+    ap.add_argument("--scheme", choices=("nnd", "weno3", "up1"), default="nnd",
+                    help="kinetic flux scheme (the reference's Iflag_scheme 1, 3, 0)")
     ap.add_argument("--tiles", type=int, default=0, help="velocity tiles per workgroup of the tile walker (0: auto)")
     ap.add_argument("--converge", type=int, default=1,
                     help="1: also measure time-to-converge on the same cavity (first-order upwind), 0: skip")
@@ -154,13 +156,16 @@ def main():
     pkg = _load("gsis_amd", PKG_DIR / "__init__.py", PKG_DIR)
     solver = _load("gsis_amd.solver", PKG_DIR / "solver.py")
     case = make_case(pkg, args.n, args.nv, args.kn, args.lid, args.dt)
+    case.const["scheme"] = {"nnd": pkg.capi.SCHEME_NND, "weno3": pkg.capi.SCHEME_WENO3,
+                            "up1": pkg.capi.SCHEME_UP1}[args.scheme]
     prec = pkg.capi.PREC_FP32 if args.precision == "fp32" else pkg.capi.PREC_FP64
     ksel = {"auto": pkg.capi.KERNEL_AUTO, "split": pkg.capi.KERNEL_SPLIT, "tiles": pkg.capi.KERNEL_TILES,
             "strip16": pkg.capi.KERNEL_STRIP16}[args.kernel]
     s = solver.GsisSolver(case, precision=prec, kstep_show=args.kstep_show, split_kernels=ksel,
                           walk_tiles=args.tiles)
     s.initialize()
-    print(f"# case {case.name}: {case.ncell} cells x {case.nv} velocities, precision {args.precision}", flush=True)
+    print(f"# case {case.name}: {case.ncell} cells x {case.nv} velocities, precision {args.precision}, "
+          f"scheme {args.scheme}", flush=True)
     s.step(args.warmup)
     s.counters()  # synchronise the library's stream
     s.timing(1)
@@ -205,7 +210,7 @@ def main():
     # workload, when a profile of it is committed
     traffic, traffic_src = None, None
     tfile = ROOT / "profiles" / f"hbm_traffic_cavity{args.n}_v{args.nv}.json"
-    if tfile.exists() and fused:
+    if tfile.exists() and fused and args.scheme == "nnd":
         rec = json.loads(tfile.read_text()).get(f"cavity{args.n}_v{args.nv}_kn{args.kn}", {})
         kern = rec.get(args.precision, {}).get("k_" + names[k])
         if kern:
@@ -225,7 +230,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain GSIS lid-driven cavity {args.n}x{args.n} cells, {args.nv}x{args.nv} sinh "
                                f"velocity grid on [-6,6]^2, Kn={args.kn}, lid {args.lid}, dt={args.dt}, "
-                               f"{args.precision}, from rest",
+                               f"{args.precision}, {args.scheme.upper()} fluxes, from rest",
                    "cells": case.ncell, "velocities": case.nv},
         "roofline": {"bound": "hbm", "kernel": names[k], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
@@ -236,7 +241,7 @@ def main():
     }
     if fp32_err is not None:
         out["fp32_rel_l2_vs_fp64"] = fp32_err
-    if args.converge and args.precision == "fp64" and args.kernel == "auto":
+    if args.converge and args.precision == "fp64" and args.kernel == "auto" and args.scheme == "nnd":
         out["time_to_converge"] = time_to_converge(pkg, solver, args)
     if args.cpu_steps > 0:
         ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -27,6 +27,7 @@ BC_PERIODIC = -5
 
 SCHEME_UP1 = 0
 SCHEME_NND = 1
+SCHEME_WENO3 = 3
 
 MODE_M0 = 0
 MODE_M1 = 1

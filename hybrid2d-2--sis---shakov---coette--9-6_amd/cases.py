@@ -70,6 +70,54 @@ def gl_velocity_grid(dv: float, nint: int, vdown: float, kgln: int):
     return np.array(nodes), np.array(weights)
 
 
+# Half-range Gauss-Hermite rules of GaussHermite (src/GKUA_lisansudu.f90:364-436):
+# the nodes and weights of  int_0^inf f(v) exp(-v^2) dv  with NV points, written
+# there as REAL(4) literals.  They are computed here from the weight function
+# itself -- the recurrence coefficients of the polynomials orthogonal on
+# [0, inf) by the Stieltjes procedure on a 96 x 48-point Gauss-Legendre
+# discretisation of exp(-v^2) over [0, 12], then the Golub-Welsch eigenvalue
+# problem -- and rounded to float32 (checked against the reference's table in
+# the tests).  NV = 8 and 16 only: the reference's 7-point table is not this
+# rule (it is exact only up to degree 6, where 7 Gauss points reach 13), so it
+# cannot be derived.
+_NGH_OK = (8, 16)
+
+
+def gauss_hermite_half_f32(nv: int):
+    """Nodes (ascending, > 0) and weights of the NV-point half-range rule,
+    rounded to float32."""
+    if nv not in _NGH_OK:
+        raise ValueError(f"half-range Gauss-Hermite NV must be one of {_NGH_OK}")
+    x, w = np.polynomial.legendre.leggauss(48)
+    edges = np.linspace(0.0, 12.0, 97)
+    a, b = edges[:-1, None], edges[1:, None]
+    X = ((a + b) / 2 + (b - a) / 2 * x).ravel()
+    W = ((b - a) / 2 * w).ravel() * np.exp(-X * X)
+    alpha, beta = np.zeros(nv), np.zeros(nv)
+    p_prev, p, nrm_prev = np.zeros_like(X), np.ones_like(X), 1.0
+    for k in range(nv):
+        nrm = float(np.sum(W * p * p))
+        alpha[k] = float(np.sum(W * X * p * p)) / nrm
+        beta[k] = nrm if k == 0 else nrm / nrm_prev
+        p_prev, p, nrm_prev = p, (X - alpha[k]) * p - beta[k] * p_prev, nrm
+    off = np.sqrt(beta[1:])
+    nodes, vec = np.linalg.eigh(np.diag(alpha) + np.diag(off, 1) + np.diag(off, -1))
+    weights = beta[0] * vec[0, :] ** 2
+    return nodes.astype(F32).astype(np.float64), weights.astype(F32).astype(np.float64)
+
+
+def gh_velocity_grid(nv: int):
+    """Gauss-Hermite grid of lisanv (src/GKUA_lisansudu.f90:103-118): the
+    half-range rule mirrored to 2*NV ascending nodes.  Returns (v, w_eff) with
+    w_eff = w exp(v^2), the per-direction factor IntegralGaussHermite applies
+    (src/GKUA_DETFP.f90:197-200; the same in GaussHermite_rs and the wall
+    integrals)."""
+    x, w = gauss_hermite_half_f32(nv)
+    v = np.concatenate([-x[::-1], x])
+    wd = np.concatenate([w[::-1], w])
+    return v, wd * np.exp(v * v)
+
+
 def newton_cotes_weights(n: int, h: float) -> np.ndarray:
     """Composite closed Newton-Cotes weights on n equally spaced points:
     Simpson 1/3 on an even number of intervals, plus Simpson 3/8 on the last
@@ -293,7 +341,9 @@ def read_quadrature_para(path: str | pathlib.Path) -> dict:
     lines = pathlib.Path(path).read_text().splitlines()
     mode = int(lines[2].split()[0])
     out = {"mode": mode}
-    if mode == 2:
+    if mode == 1:
+        out["ngh"] = {1: 7, 2: 8, 3: 16}[int(lines[6].split()[0])]
+    elif mode == 2:
         dvx, nvi, vxd, vxu = lines[9].split()[:4]
         dvy, nvj, vyd, vyu = lines[11].split()[:4]
         out.update(dvx=float(dvx), nvi=int(nvi), vxdown=float(vxd), vxup=float(vxu),
@@ -397,10 +447,14 @@ def shipped_case(input_dir: str | pathlib.Path, name: str = "shipped") -> Case:
     bcs = read_generic_inp(d / "generic.inp", nx, ny)
     mesh = build_mesh(xn, yn, bcs, ctl["Lref"])
     q = read_quadrature_para(d / "Quadrature_Para.dat")
-    if q["mode"] != 2:
-        raise NotImplementedError("only Gauss-Legendre (mode 2) input is implemented")
-    vi, wi = gl_velocity_grid(q["dvx"], q["nvi"], q["vxdown"], q["kgln"])
-    vj, wj = gl_velocity_grid(q["dvy"], q["nvj"], q["vydown"], q["kgln"])
+    if q["mode"] == 1:
+        vi, wi = gh_velocity_grid(q["ngh"])
+        vj, wj = vi.copy(), wi.copy()
+    elif q["mode"] == 2:
+        vi, wi = gl_velocity_grid(q["dvx"], q["nvi"], q["vxdown"], q["kgln"])
+        vj, wj = gl_velocity_grid(q["dvy"], q["nvj"], q["vydown"], q["kgln"])
+    else:
+        raise NotImplementedError("Newton-Cotes input (mode 3) stops in the reference (src/GKUA_DETFP.f90:40-43)")
     return Case(name, mesh, bcs, vi, wi, vj, wj, constants_from_control(ctl), "freestream")
 
 
@@ -416,6 +470,8 @@ def cavity_case(n: int, nv: int, kn: float, lid: float = 0.15, vmax: float = 6.0
     mesh = build_mesh(xn, yn, bcs)
     if grid == "sinh":
         v, w = sinh_velocity_grid(nv, vmax)
+    elif grid == "gh":
+        v, w = gh_velocity_grid(nv // 2)  # nv = 2 NV nodes per direction
     else:
         v, w = nc_velocity_grid(nv, vmax)
     const = constants_from_kn(kn, dt=dt)

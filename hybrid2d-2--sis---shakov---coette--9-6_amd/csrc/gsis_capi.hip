@@ -91,8 +91,8 @@ int validate(const gsis_params *p) {
       !p->ni2 || !p->nj1 || !p->nj2)
     return fail(GSIS_EARG, "a required array pointer is NULL");
   if (p->nbc < 0 || (p->nbc > 0 && !p->bc)) return fail(GSIS_EARG, "bad BC list");
-  if (p->scheme != GSIS_SCHEME_UP1 && p->scheme != GSIS_SCHEME_NND)
-    return fail(GSIS_EUNSUPPORTED, "only the UP1 and NND kinetic schemes are implemented");
+  if (p->scheme != GSIS_SCHEME_UP1 && p->scheme != GSIS_SCHEME_NND && p->scheme != GSIS_SCHEME_WENO3)
+    return fail(GSIS_EUNSUPPORTED, "kinetic scheme must be GSIS_SCHEME_UP1, GSIS_SCHEME_NND or GSIS_SCHEME_WENO3");
   if (p->precision != GSIS_PREC_FP64 && p->precision != GSIS_PREC_FP32)
     return fail(GSIS_EARG, "precision must be GSIS_PREC_FP64 or GSIS_PREC_FP32");
   if (p->mode != GSIS_MODE_M0 && p->mode != GSIS_MODE_M1) return fail(GSIS_EARG, "unknown mode");

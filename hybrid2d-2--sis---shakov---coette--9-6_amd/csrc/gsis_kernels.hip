@@ -25,6 +25,47 @@ template <typename T>
 __device__ __forceinline__ T posp(T a) { return fmax(a, T(0)); }
 template <typename T>
 __device__ __forceinline__ T negp(T a) { return fmin(a, T(0)); }
+// 1/a for a > 0 (the LU-SGS diagonal, the WENO3 weight sum): hardware reciprocal estimate and two
+// Newton-Raphson steps r <- r + r(1 - a r), accurate to about 1 ulp
+__device__ __forceinline__ double rcp_pos(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = fma(-a, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-a, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_pos(float a) {
+  float r = __builtin_amdgcn_rcpf(a);
+  const float e = fmaf(-a, r, 1.0f);
+  return fmaf(r, e, r);
+}
+
+// WENO3 reconstruction (RHSFDVsub, src/GKUA_Residual.f90:167-185 and the
+// wall-adjacent faces :202-250, :333-380).  With D_k the cell differences of
+// the stencil, IS_k = D_k^2, alpha_k = C_k / (IS_k + eps)^2 and the
+// normalised weights omega_k, the reference's two states are
+//   UL = F(f-1) + 1/2 (omega_1 D(f-1) + omega_2 D(f))    (C_1 = 1/3, C_2 = 2/3)
+//   UR = F(f)   - 1/2 (omega_1 D(f)   + omega_2 D(f+1))  (C_1 = 2/3, C_2 = 1/3)
+// i.e. the NND form with the minmod slope replaced by a weighted mean of the
+// difference across the face (dc, weight 2/3) and the outer one (dn, 1/3).
+// Clearing the denominators (the REAL(4) literals 2./3. and 1./3. differ by
+// exactly a factor 2) leaves one division:
+//   slope = (bc dn + 2 bn dc) / (bc + 2 bn),  b = (D^2 + eps)^2.
+// eps is the REAL(4) literal 1.E-15 (src/GKUA_Residual.f90:143).
+constexpr int kSchemeWeno3 = 3;  // Iflag_scheme of WENO3 (src/COMMONDATA.f90:18)
+constexpr double kEpsWeno = 1.0000000036274937e-15;  // (double)1.0e-15f
+template <typename T>
+__device__ __forceinline__ T wslope(T dc, T dn) {
+  const T sc = fma(dc, dc, T(kEpsWeno)), sn = fma(dn, dn, T(kEpsWeno));
+  const T bc = sc * sc, bn2 = T(2) * (sn * sn);
+  return fma(bc, dn, bn2 * dc) * rcp_pos(bc + bn2);
+}
+// limited slope of the NND (SL = 0: minmod) or WENO3 (SL = 1) state
+template <int SL, typename T>
+__device__ __forceinline__ T slope(T dc, T dn) {
+  if constexpr (SL == 0) return fminmd(dc, dn);
+  else return wslope(dc, dn);
+}
 
 // Block-wide sum of N doubles (wave64 shuffles, then one LDS pass).  All
 // threads receive the totals.  blockDim.x must be a multiple of 64, <= 1024.
@@ -260,63 +301,65 @@ __global__ void k_periodic(Dev d, BcDev b, T *FG, T *FH) {
 // of the RHS per cell (GaussLegendre_rs, src/GKUA_DETFP_RES.f90:81-120).
 // One block per interior cell; threads stride over velocities.
 // ------------------------------------------------------------------------
+// sch: GSIS scheme code (0 UP1, 1 NND4A2, 3 WENO3), uniform over the launch
 template <typename T>
-__device__ __forceinline__ T flux_nnd(T a, T fm2, T fm1, T f0, T fp1, bool nnd) {
+__device__ __forceinline__ T flux_nnd(T a, T fm2, T fm1, T f0, T fp1, int sch) {
   const T asp = posp(a), asn = negp(a);
-  if (!nnd) return asp * fm1 + asn * f0;
+  if (sch == 0) return asp * fm1 + asn * f0;
   const T d0 = f0 - fm1;
-  const T ulg = fm1 + T(0.5) * fminmd(d0, fm1 - fm2);
-  const T urg = f0 - T(0.5) * fminmd(d0, fp1 - f0);
-  return asp * ulg + asn * urg;
+  const T sl = sch == 3 ? slope<1>(d0, fm1 - fm2) : slope<0>(d0, fm1 - fm2);
+  const T sr = sch == 3 ? slope<1>(d0, fp1 - f0) : slope<0>(d0, fp1 - f0);
+  return asp * (fm1 + T(0.5) * sl) + asn * (f0 - T(0.5) * sr);
 }
-// face next to a wall/farfield face on the low side: unlimited left state
-// (src/GKUA_Residual.f90:214-220)
+// face next to a wall/farfield face on the low side: unlimited (central)
+// left state (src/GKUA_Residual.f90:202-220)
 template <typename T>
-__device__ __forceinline__ T flux_lowadj(T a, T f1, T f2, T f3, bool nnd) {
+__device__ __forceinline__ T flux_lowadj(T a, T f1, T f2, T f3, int sch) {
   const T asp = posp(a), asn = negp(a);
-  if (!nnd) return asp * f1 + asn * f2;
+  if (sch == 0) return asp * f1 + asn * f2;
   const T ulg = f1 + T(0.5) * (f2 - f1);
-  const T urg = f2 - T(0.5) * fminmd(f2 - f1, f3 - f2);
-  return asp * ulg + asn * urg;
+  const T sr = sch == 3 ? slope<1>(f2 - f1, f3 - f2) : slope<0>(f2 - f1, f3 - f2);
+  return asp * ulg + asn * (f2 - T(0.5) * sr);
 }
-// face next to a wall/farfield face on the high side: unlimited right state
-// (src/GKUA_Residual.f90:251-257); f1 = F(n-1), f2 = F(n-2), f3 = F(n-3)
+// face next to a wall/farfield face on the high side: unlimited (central)
+// right state (src/GKUA_Residual.f90:239-257); f1 = F(n-1), f2 = F(n-2),
+// f3 = F(n-3)
 template <typename T>
-__device__ __forceinline__ T flux_highadj(T a, T f1, T f2, T f3, bool nnd) {
+__device__ __forceinline__ T flux_highadj(T a, T f1, T f2, T f3, int sch) {
   const T asp = posp(a), asn = negp(a);
-  if (!nnd) return asp * f2 + asn * f1;
-  const T ulg = f2 + T(0.5) * fminmd(f1 - f2, f2 - f3);
+  if (sch == 0) return asp * f2 + asn * f1;
+  const T sl = sch == 3 ? slope<1>(f1 - f2, f2 - f3) : slope<0>(f1 - f2, f2 - f3);
   const T urg = f1 - T(0.5) * (f1 - f2);
-  return asp * ulg + asn * urg;
+  return asp * (f2 + T(0.5) * sl) + asn * urg;
 }
 
 // flux through i-face fi of row j (between cells fi-1 and fi); stencil
 // F(fi-2), F(fi-1), F(fi), F(fi+1)
 template <typename T>
 __device__ __forceinline__ T iface_flux(const Dev &d, int fi, int j, T a, T fm2, T fm1, T f0, T fp1,
-                                        const T *slot, int ird, int v, bool nnd) {
+                                        const T *slot, int ird, int v, int sch) {
   if (d.ovrI1[j]) {
     if (fi == 1) return a * slot[slot_idx(d, 1, j + 1, ird, v)];
-    if (fi == 2) return flux_lowadj(a, fm1, f0, fp1, nnd);
+    if (fi == 2) return flux_lowadj(a, fm1, f0, fp1, sch);
   }
   if (d.ovrI3[j]) {
     if (fi == d.nx) return a * slot[slot_idx(d, 3, j + 1, ird, v)];
-    if (fi == d.nx - 1) return flux_highadj(a, f0, fm1, fm2, nnd);
+    if (fi == d.nx - 1) return flux_highadj(a, f0, fm1, fm2, sch);
   }
-  return flux_nnd(a, fm2, fm1, f0, fp1, nnd);
+  return flux_nnd(a, fm2, fm1, f0, fp1, sch);
 }
 template <typename T>
 __device__ __forceinline__ T jface_flux(const Dev &d, int i, int fj, T a, T fm2, T fm1, T f0, T fp1,
-                                        const T *slot, int ird, int v, bool nnd) {
+                                        const T *slot, int ird, int v, int sch) {
   if (d.ovrJ2[i]) {
     if (fj == 1) return a * slot[slot_idx(d, 2, i + 1, ird, v)];
-    if (fj == 2) return flux_lowadj(a, fm1, f0, fp1, nnd);
+    if (fj == 2) return flux_lowadj(a, fm1, f0, fp1, sch);
   }
   if (d.ovrJ4[i]) {
     if (fj == d.ny) return a * slot[slot_idx(d, 4, i + 1, ird, v)];
-    if (fj == d.ny - 1) return flux_highadj(a, f0, fm1, fm2, nnd);
+    if (fj == d.ny - 1) return flux_highadj(a, f0, fm1, fm2, sch);
   }
-  return flux_nnd(a, fm2, fm1, f0, fp1, nnd);
+  return flux_nnd(a, fm2, fm1, f0, fp1, sch);
 }
 
 template <typename T>
@@ -347,7 +390,7 @@ __global__ void __launch_bounds__(256) k_rhs(Dev d, const T *FG, const T *FH, co
   const double sjb = d.sj[gidx(d, i, j)], sjt = d.sj[gidx(d, i, j + 1)];
   const double njb1 = d.nj1[gidx(d, i, j)], njb2 = d.nj2[gidx(d, i, j)];
   const double njt1 = d.nj1[gidx(d, i, j + 1)], njt2 = d.nj2[gidx(d, i, j + 1)];
-  const bool nnd = d.scheme != 0;
+  const int sch = d.scheme;
   const size_t nv = d.nv;
   const size_t o0 = (size_t)c * nv;
   const size_t oim2 = (size_t)cidx(d, i - 2, j) * nv, oim1 = (size_t)cidx(d, i - 1, j) * nv;
@@ -374,10 +417,10 @@ __global__ void __launch_bounds__(256) k_rhs(Dev d, const T *FG, const T *FH, co
       const T f0 = F[o0 + v];
       const T fim2 = F[oim2 + v], fim1 = F[oim1 + v], fip1 = F[oip1 + v], fip2 = F[oip2 + v];
       const T fjm2 = F[ojm2 + v], fjm1 = F[ojm1 + v], fjp1 = F[ojp1 + v], fjp2 = F[ojp2 + v];
-      const T flx_l = iface_flux(d, i, j, axl, fim2, fim1, f0, fip1, slot, ird, v, nnd);
-      const T flx_r = iface_flux(d, i + 1, j, axr, fim1, f0, fip1, fip2, slot, ird, v, nnd);
-      const T fly_b = jface_flux(d, i, j, ayb, fjm2, fjm1, f0, fjp1, slot, ird, v, nnd);
-      const T fly_t = jface_flux(d, i, j + 1, ayt, fjm1, f0, fjp1, fjp2, slot, ird, v, nnd);
+      const T flx_l = iface_flux(d, i, j, axl, fim2, fim1, f0, fip1, slot, ird, v, sch);
+      const T flx_r = iface_flux(d, i + 1, j, axr, fim1, f0, fip1, fip2, slot, ird, v, sch);
+      const T fly_b = jface_flux(d, i, j, ayb, fjm2, fjm1, f0, fjp1, slot, ird, v, sch);
+      const T fly_t = jface_flux(d, i, j + 1, ayt, fjm1, f0, fjp1, fjp2, slot, ird, v, sch);
       const T net = ((flx_r - flx_l) + fly_t) - fly_b;
       const T feq = ird == 0 ? feqg : feqh;
       rhs[ird] = -net / vol + cf * (feq - f0);
@@ -513,23 +556,24 @@ __global__ void k_cellpar(Dev d, double *cpar, int pstride) {
 // sv is the wall-slot value of the boundary face.  Next to an overridden face
 // the limiter on that side is dropped (flux_lowadj / flux_highadj), on it the
 // flux is a * slot (src/GKUA_Residual.f90:200-262).
-// hs is 1/2 for the NND scheme and 0 for first-order upwind (the limited
-// slopes then drop out exactly).
-template <typename T>
+// hs is 1/2 for the NND and WENO3 schemes and 0 for first-order upwind (the
+// limited slopes then drop out exactly); SL picks the slope rule (0 minmod,
+// 1 WENO3).
+template <int SL, typename T>
 __device__ __forceinline__ T wflux(int f, int n, int fl, T a, T fm2, T fm1, T f0, T fp1, T sv, T hs) {
   const T asp = posp(a), asn = negp(a);
   const T d0 = f0 - fm1;
-  const T liml = ((fl & 1) && f == 2) ? d0 : fminmd(d0, fm1 - fm2);
-  const T limr = ((fl & 2) && f == n - 1) ? d0 : fminmd(d0, fp1 - f0);
+  const T liml = ((fl & 1) && f == 2) ? d0 : slope<SL>(d0, fm1 - fm2);
+  const T limr = ((fl & 2) && f == n - 1) ? d0 : slope<SL>(d0, fp1 - f0);
   const T fx = asp * (fm1 + hs * liml) + asn * (f0 - hs * limr);
   const bool wall = ((fl & 1) && f == 1) || ((fl & 2) && f == n);
   return wall ? a * sv : fx;
 }
 // interior face (no override) of the fused kernel's fast path
-template <typename T>
+template <int SL, typename T>
 __device__ __forceinline__ T iflux(T a, T fm2, T fm1, T f0, T fp1, T hs) {
   const T d0 = f0 - fm1;
-  return posp(a) * (fm1 + hs * fminmd(d0, fm1 - fm2)) + negp(a) * (f0 - hs * fminmd(d0, fp1 - f0));
+  return posp(a) * (fm1 + hs * slope<SL>(d0, fm1 - fm2)) + negp(a) * (f0 - hs * slope<SL>(d0, fp1 - f0));
 }
 // exp(x) for the equilibrium, x <= 0.  Cody-Waite reduction x = k ln2 + r
 // (|r| <= ln2/2, ln2 split into the fdlibm high/low parts so that k*ln2_hi is
@@ -566,20 +610,6 @@ __device__ __forceinline__ double wexp(double x) {
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
   return ldexp(p, (int)k);
-}
-// 1/a for a > 0 (the LU-SGS diagonal): hardware reciprocal estimate and two
-// Newton-Raphson steps r <- r + r(1 - a r), accurate to about 1 ulp
-__device__ __forceinline__ double rcp_pos(double a) {
-  double r = __builtin_amdgcn_rcp(a);
-  double e = fma(-a, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-a, r, 1.0);
-  return fma(r, e, r);
-}
-__device__ __forceinline__ float rcp_pos(float a) {
-  float r = __builtin_amdgcn_rcpf(a);
-  const float e = fmaf(-a, r, 1.0f);
-  return fmaf(r, e, r);
 }
 __device__ __forceinline__ float wexp(float x) { return expf(x); }
 // element at a 32-bit byte offset from a uniform base (global_load with an
@@ -619,7 +649,7 @@ __device__ __forceinline__ void sto(T *base, unsigned off, T v) {
 // plane and the wall slots (slot at byte offset slotb) in one allocation of
 // less than 4 GiB.
 // ------------------------------------------------------------------------
-template <typename T>
+template <typename T, int SL>
 __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT, int K, T *F, unsigned planeb,
                                                unsigned slotb, T *R, int store_rhs) {
   using T2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
@@ -761,7 +791,7 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
     }
     // Interior fast path, taken when every lane of the wavefront is at an
     // interior position (no face override, no wall slot or ghost value
-    // within reach, finalised row in the same tile): plain NND fluxes and no
+    // within reach, finalised row in the same tile): plain NND (or WENO3) fluxes and no
     // boundary loads.  Everything else takes the general path.
     const bool inner = active && r >= 3 && r <= nrow - 2 && i >= 2 && i <= ncol - 2;
     if (__all(inner)) {
@@ -793,8 +823,8 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
       const T ayb = vx * T(cc[16]) + vy * T(cc[17]);
       const T ayt = vx * T(cc[18]) + vy * T(cc[19]);
       const T2 xl = X[(pbuf * ncol + col - 1) * VT + lane];
-      const T flxr = iflux(axr, fm1, w2, fp1, fp2, hs);
-      const T flyt = iflux(ayt, w1, w2, w3, w4, hs);
+      const T flxr = iflux<SL>(axr, fm1, w2, fp1, fp2, hs);
+      const T flyt = iflux<SL>(ayt, w1, w2, w3, w4, hs);
       const T rr = -(((flxr - xl.x) + flyt) - fly_prev) * T(cc[9]) + T(cc[8]) * (feq - w2);
       fly_prev = flyt;
       // forward recursion (src/GKUA_Residual.f90:39-57)
@@ -864,17 +894,17 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
         const T axr = vx * T(cc[14]) + vy * T(cc[15]);
         T flxl, dl = T(0);
         if (i == 1) {
-          flxl = wflux(1, nx, rf, axl, xs, fm1, w2, fp1, xs, hs);
+          flxl = wflux<SL>(1, nx, rf, axl, xs, fm1, w2, fp1, xs, hs);
         } else {
           const T2 xl = X[(pbuf * ncol + col - 1) * VT + lane];
           flxl = xl.x;
           dl = xl.y;
         }
-        const T flxr = wflux(i + 1, nx, rf, axr, fm1, w2, fp1, fp2, xs, hs);
+        const T flxr = wflux<SL>(i + 1, nx, rf, axr, fm1, w2, fp1, fp2, xs, hs);
         const T ayb = vx * T(cc[16]) + vy * T(cc[17]);
         const T ayt = vx * T(cc[18]) + vy * T(cc[19]);
-        const T flyb = first ? wflux(1, ny, cf, ayb, w0, w1, w2, w3, ys, hs) : fly_prev;
-        const T flyt = wflux(r + 1, ny, cf, ayt, w1, w2, w3, w4, ys, hs);
+        const T flyb = first ? wflux<SL>(1, ny, cf, ayb, w0, w1, w2, w3, ys, hs) : fly_prev;
+        const T flyt = wflux<SL>(r + 1, ny, cf, ayt, w1, w2, w3, w4, ys, hs);
         fly_prev = flyt;
         const T rr = -(((flxr - flxl) + flyt) - flyb) * T(cc[9]) + T(cc[8]) * (feq - w2);
         if (store_rhs && act) sto(R, tb + (unsigned)(r + 1) * rb + (unsigned)(i + 1) * cb, rr);
@@ -973,8 +1003,11 @@ __device__ __forceinline__ void lds_copy16(const void *src, void *dst) {
 #endif
 }
 
-template <typename T, int NW>
-__global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restrict__ cpar, T *__restrict__ F,
+// SL: slope rule of the fluxes (0 minmod for NND / UP1, 1 WENO3).  At least
+// 4 wavefronts per SIMD (at most 128 VGPRs): with 80 KB of LDS per 8-column
+// workgroup two workgroups share a CU, which the register file must allow.
+template <typename T, int NW, int SL>
+__global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const double *__restrict__ cpar, T *__restrict__ F,
                                                  const T *__restrict__ slot, T *__restrict__ R, int store_rhs,
                                                  int ntiles, T *msg, T *dright, int *flag, int epoch, int *werr,
                                                  const int *__restrict__ oflags) {
@@ -1187,8 +1220,8 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
         // left face: boundary flux, the seam message or column i-1's value
         T gfl, gdl, hfl, hdl;
         if (xlo) {
-          gfl = wflux(1, nx, rfl, axl, cgxs, cgm1, g2, cgp1, cgxs, hs);
-          hfl = wflux(1, nx, rfl, axl, chxs, chm1, h2, chp1, chxs, hs);
+          gfl = wflux<SL>(1, nx, rfl, axl, cgxs, cgm1, g2, cgp1, cgxs, hs);
+          hfl = wflux<SL>(1, nx, rfl, axl, chxs, chm1, h2, chp1, chxs, hs);
           gdl = T(0);
           hdl = T(0);
         } else if (w == 0) {
@@ -1199,22 +1232,22 @@ __global__ void __launch_bounds__(64 * NW) k_strip(Dev d, const double *__restri
         }
         T gfr, hfr, gft, hft, gfb, hfb;
         if (xrule) {
-          gfr = wflux(i + 1, nx, rfl, axr, cgm1, g2, cgp1, cgp2, cgxs, hs);
-          hfr = wflux(i + 1, nx, rfl, axr, chm1, h2, chp1, chp2, chxs, hs);
+          gfr = wflux<SL>(i + 1, nx, rfl, axr, cgm1, g2, cgp1, cgp2, cgxs, hs);
+          hfr = wflux<SL>(i + 1, nx, rfl, axr, chm1, h2, chp1, chp2, chxs, hs);
         } else {
-          gfr = iflux(axr, cgm1, g2, cgp1, cgp2, hs);
-          hfr = iflux(axr, chm1, h2, chp1, chp2, hs);
+          gfr = iflux<SL>(axr, cgm1, g2, cgp1, cgp2, hs);
+          hfr = iflux<SL>(axr, chm1, h2, chp1, chp2, hs);
         }
         if (yrule) {
-          gft = wflux(r + 1, ny, cfl, ayt, g1, g2, g3, g4, cgys, hs);
-          hft = wflux(r + 1, ny, cfl, ayt, h1, h2, h3, h4, chys, hs);
+          gft = wflux<SL>(r + 1, ny, cfl, ayt, g1, g2, g3, g4, cgys, hs);
+          hft = wflux<SL>(r + 1, ny, cfl, ayt, h1, h2, h3, h4, chys, hs);
         } else {
-          gft = iflux(ayt, g1, g2, g3, g4, hs);
-          hft = iflux(ayt, h1, h2, h3, h4, hs);
+          gft = iflux<SL>(ayt, g1, g2, g3, g4, hs);
+          hft = iflux<SL>(ayt, h1, h2, h3, h4, hs);
         }
         if (first) {
-          gfb = wflux(1, ny, cfl, ayb, g0, g1, g2, g3, cgys, hs);
-          hfb = wflux(1, ny, cfl, ayb, h0, h1, h2, h3, chys, hs);
+          gfb = wflux<SL>(1, ny, cfl, ayb, g0, g1, g2, g3, cgys, hs);
+          hfb = wflux<SL>(1, ny, cfl, ayb, h0, h1, h2, h3, chys, hs);
         } else {
           gfb = gfly;
           hfb = hfly;
@@ -1746,8 +1779,10 @@ bool walk_config(const Dev &d, int tiles, int *vt_out, int *k_out, size_t *lds_o
   if (lds > (size_t)max_lds) return false;
   static bool attr_set = false;
   if (lds > 65536 && !attr_set) {
-    if (hipFuncSetAttribute((const void *)k_walk<T>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds) !=
-        hipSuccess)
+    if (hipFuncSetAttribute((const void *)k_walk<T, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void *)k_walk<T, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds) !=
+            hipSuccess)
       return false;
     attr_set = true;
   }
@@ -1769,8 +1804,14 @@ bool launch_strip_nw(const Dev &d, double *cpar, T *F, const T *slot, T *R, int 
   hipLaunchKernelGGL(k_cellpar, dim3((ncl + 255) / 256), dim3(256), 0, s, d, cpar, 0);
   b.epoch += 1;
   if (b.ev_before) (void)hipEventRecord(b.ev_before, s);
-  hipLaunchKernelGGL((k_strip<T, NW>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cpar, F, slot, R, store_rhs,
-                     ntiles, (T *)b.msg, (T *)b.dright, b.flag, b.epoch, b.err, b.oflags);
+  // the WENO3 scheme is a separate instantiation so that the NND / UP1
+  // kernel's register allocation is untouched by the WENO3 weights
+  if (d.scheme == kSchemeWeno3)
+    hipLaunchKernelGGL((k_strip<T, NW, 1>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cpar, F, slot, R,
+                       store_rhs, ntiles, (T *)b.msg, (T *)b.dright, b.flag, b.epoch, b.err, b.oflags);
+  else
+    hipLaunchKernelGGL((k_strip<T, NW, 0>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cpar, F, slot, R,
+                       store_rhs, ntiles, (T *)b.msg, (T *)b.dright, b.flag, b.epoch, b.err, b.oflags);
   if (b.ev_after) (void)hipEventRecord(b.ev_after, s);
   if (ngroups > 1) {
     const size_t nthr = rows * 64;
@@ -1806,8 +1847,12 @@ bool launch_walk(const Dev &d, double *cpar, T *F, const T *slot, T *R, int stor
   const int ntiles = 2 * ((d.nv + vt - 1) / vt);
   const int nb = (ntiles + K - 1) / K;
   const unsigned planeb = (unsigned)((size_t)d.ncell * d.nv * sizeof(T));
-  hipLaunchKernelGGL(k_walk<T>, dim3(nb), dim3(vt * (d.nx - 1)), lds, s, d, cpar, vt, K, F, planeb, (unsigned)so, R,
-                     store_rhs);
+  if (d.scheme == kSchemeWeno3)
+    hipLaunchKernelGGL((k_walk<T, 1>), dim3(nb), dim3(vt * (d.nx - 1)), lds, s, d, cpar, vt, K, F, planeb,
+                       (unsigned)so, R, store_rhs);
+  else
+    hipLaunchKernelGGL((k_walk<T, 0>), dim3(nb), dim3(vt * (d.nx - 1)), lds, s, d, cpar, vt, K, F, planeb,
+                       (unsigned)so, R, store_rhs);
   return true;
 }
 

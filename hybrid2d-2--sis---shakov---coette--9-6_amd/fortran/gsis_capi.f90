@@ -15,7 +15,7 @@ module gsis_capi
   integer(c_int), parameter :: GSIS_OK = 0, GSIS_EARG = 1, GSIS_EHIP = 2, GSIS_ENAN = 3, &
                                GSIS_EUNSUPPORTED = 4, GSIS_ETEMP = 5
   integer(c_int32_t), parameter :: GSIS_BC_WALL = -2, GSIS_BC_FARFIELD = -4, GSIS_BC_PERIODIC = -5
-  integer(c_int32_t), parameter :: GSIS_SCHEME_UP1 = 0, GSIS_SCHEME_NND = 1
+  integer(c_int32_t), parameter :: GSIS_SCHEME_UP1 = 0, GSIS_SCHEME_NND = 1, GSIS_SCHEME_WENO3 = 3
   integer(c_int32_t), parameter :: GSIS_MODE_M0 = 0, GSIS_MODE_M1 = 1
   integer(c_int32_t), parameter :: GSIS_PREC_FP64 = 0, GSIS_PREC_FP32 = 1
   integer(c_int32_t), parameter :: GSIS_KERNEL_AUTO = 0, GSIS_KERNEL_SPLIT = 1, GSIS_KERNEL_TILES = 2, &

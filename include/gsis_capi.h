@@ -44,6 +44,7 @@ extern "C" {
 /* kinetic flux schemes (src/COMMONDATA.f90:18) */
 #define GSIS_SCHEME_UP1 0
 #define GSIS_SCHEME_NND 1
+#define GSIS_SCHEME_WENO3 3 /* third-order WENO, src/GKUA_Residual.f90:167-185 */
 
 /* step modes */
 #define GSIS_MODE_M0 0 /* the step as shipped: DVM implicit step + HoT, synthetic solve off */

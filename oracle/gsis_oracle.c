@@ -380,13 +380,37 @@ typedef struct {
 #define WAY(w, i, j) (w)->ay[(i) + (size_t)WNI * (j)]
 #define WD(w, ird, k) (w)->dfrd[((ird)-1) + 2 * ((k) + 1)]
 
-/* RHSFDVsub, src/GKUA_Residual.f90:117-418 (UP1 and NND4A2 schemes) */
+/* WENO3 constants of RHSFDVsub (src/GKUA_Residual.f90:143-144): the
+ * literals are REAL(4), converted to the REAL(8) variables they are stored in */
+static const double EPS_WENO = (double)1.0e-15f;
+static const double C_WENO1 = (double)(1.0f / 3.0f), C_WENO2 = (double)(2.0f / 3.0f);
+
+/* WENO3 left state at a face from F(f-2), F(f-1), F(f) with smoothness
+ * indicators is1 = (F(f-1)-F(f-2))^2, is2 = (F(f)-F(f-1))^2
+ * (src/GKUA_Residual.f90:170-175) */
+static double weno_left(double fm2, double fm1, double f0, double is1, double is2) {
+  double a1 = C_WENO1 / ((is1 + EPS_WENO) * (is1 + EPS_WENO));
+  double a2 = C_WENO2 / ((is2 + EPS_WENO) * (is2 + EPS_WENO));
+  double as = a1 + a2, o1 = a1 / as, o2 = a2 / as;
+  return o1 * (-0.5 * fm2 + 1.5 * fm1) + o2 * (0.5 * fm1 + 0.5 * f0);
+}
+/* WENO3 right state at a face from F(f-1), F(f), F(f+1) with
+ * is2 = (F(f)-F(f-1))^2, is3 = (F(f+1)-F(f))^2 (src/GKUA_Residual.f90:177-182) */
+static double weno_right(double fm1, double f0, double fp1, double is2, double is3) {
+  double a1 = C_WENO2 / ((is2 + EPS_WENO) * (is2 + EPS_WENO));
+  double a2 = C_WENO1 / ((is3 + EPS_WENO) * (is3 + EPS_WENO));
+  double as = a1 + a2, o1 = a1 / as, o2 = a2 / as;
+  return o1 * (0.5 * fm1 + 0.5 * f0) + o2 * (1.5 * f0 - 0.5 * fp1);
+}
+
+/* RHSFDVsub, src/GKUA_Residual.f90:117-418 (UP1, NND4A2 and WENO3 schemes) */
 static void rhs_velocity(const orc_t *o, work_t *w, int iv, int jv) {
   const int nx = o->nx, ny = o->ny;
   const double *Fv = fvel(o, o->F, iv, jv);
   const double *F0v = fvel(o, o->F0, iv, jv);
   const double vx = o->vi[iv - 1], vy = o->vj[jv - 1];
-  const int nnd = (o->scheme != GSIS_SCHEME_UP1);
+  const int nnd = (o->scheme == GSIS_SCHEME_NND);
+  const int weno = (o->scheme == GSIS_SCHEME_WENO3);
   /* i-direction (:145-187) */
   for (int j = 1; j <= ny - 1; j++) {
     for (int i = 0; i <= nx + 1; i++)
@@ -396,7 +420,13 @@ static void rhs_velocity(const orc_t *o, work_t *w, int iv, int jv) {
       WAX(w, i, j) = a;
       double asp = (a + fabs(a)) * 0.5, asn = (a - fabs(a)) * 0.5;
       for (int r = 1; r <= 2; r++) {
-        if (!nnd) {
+        if (weno) { /* :167-185 */
+          double is1 = WD(w, r, i - 1) * WD(w, r, i - 1), is2 = WD(w, r, i) * WD(w, r, i);
+          double is3 = WD(w, r, i + 1) * WD(w, r, i + 1);
+          double ulh = weno_left(FV(Fv, r, i - 2, j), FV(Fv, r, i - 1, j), FV(Fv, r, i, j), is1, is2);
+          double urh = weno_right(FV(Fv, r, i - 1, j), FV(Fv, r, i, j), FV(Fv, r, i + 1, j), is2, is3);
+          WFL(w, r, i, j) = asp * ulh + asn * urh;
+        } else if (!nnd) {
           WFL(w, r, i, j) = asp * FV(Fv, r, i - 1, j) + asn * FV(Fv, r, i, j);
         } else {
           double ulg = FV(Fv, r, i - 1, j) + 0.5 * fminmd(WD(w, r, i), WD(w, r, i - 1));
@@ -415,7 +445,10 @@ static void rhs_velocity(const orc_t *o, work_t *w, int iv, int jv) {
         double a = WAX(w, 2, j), asp = (a + fabs(a)) * 0.5, asn = (a - fabs(a)) * 0.5;
         for (int r = 1; r <= 2; r++) {
           double f1 = FV(Fv, r, 1, j), f2 = FV(Fv, r, 2, j), f3 = FV(Fv, r, 3, j);
-          if (!nnd) WFL(w, r, 2, j) = asp * f1 + asn * f2;
+          if (weno) { /* central left state, WENO right state (:202-213) */
+            double urh = weno_right(f1, f2, f3, (f2 - f1) * (f2 - f1), (f3 - f2) * (f3 - f2));
+            WFL(w, r, 2, j) = asp * (0.5 * (f1 + f2)) + asn * urh;
+          } else if (!nnd) WFL(w, r, 2, j) = asp * f1 + asn * f2;
           else {
             double ulg = f1 + 0.5 * (f2 - f1);
             double urg = f2 - 0.5 * fminmd(f2 - f1, f3 - f2);
@@ -429,7 +462,10 @@ static void rhs_velocity(const orc_t *o, work_t *w, int iv, int jv) {
         double a = WAX(w, nx - 1, j), asp = (a + fabs(a)) * 0.5, asn = (a - fabs(a)) * 0.5;
         for (int r = 1; r <= 2; r++) {
           double f1 = FV(Fv, r, nx - 1, j), f2 = FV(Fv, r, nx - 2, j), f3 = FV(Fv, r, nx - 3, j);
-          if (!nnd) WFL(w, r, nx - 1, j) = asp * f2 + asn * f1;
+          if (weno) { /* WENO left state, central right state (:239-250) */
+            double ulh = weno_left(f3, f2, f1, (f2 - f3) * (f2 - f3), (f1 - f2) * (f1 - f2));
+            WFL(w, r, nx - 1, j) = asp * ulh + asn * (0.5 * f2 + 0.5 * f1);
+          } else if (!nnd) WFL(w, r, nx - 1, j) = asp * f2 + asn * f1;
           else {
             double ulg = f2 + 0.5 * fminmd(f1 - f2, f2 - f3);
             double urg = f1 - 0.5 * (f1 - f2);
@@ -452,7 +488,13 @@ static void rhs_velocity(const orc_t *o, work_t *w, int iv, int jv) {
       WAY(w, i, j) = a;
       double asp = (a + fabs(a)) * 0.5, asn = (a - fabs(a)) * 0.5;
       for (int r = 1; r <= 2; r++) {
-        if (!nnd) {
+        if (weno) { /* :300-318 */
+          double is1 = WD(w, r, j - 1) * WD(w, r, j - 1), is2 = WD(w, r, j) * WD(w, r, j);
+          double is3 = WD(w, r, j + 1) * WD(w, r, j + 1);
+          double ulh = weno_left(FV(Fv, r, i, j - 2), FV(Fv, r, i, j - 1), FV(Fv, r, i, j), is1, is2);
+          double urh = weno_right(FV(Fv, r, i, j - 1), FV(Fv, r, i, j), FV(Fv, r, i, j + 1), is2, is3);
+          WFL(w, r, i, j) = asp * ulh + asn * urh;
+        } else if (!nnd) {
           WFL(w, r, i, j) = asp * FV(Fv, r, i, j - 1) + asn * FV(Fv, r, i, j);
         } else {
           double ulg = FV(Fv, r, i, j - 1) + 0.5 * fminmd(WD(w, r, j), WD(w, r, j - 1));
@@ -471,7 +513,10 @@ static void rhs_velocity(const orc_t *o, work_t *w, int iv, int jv) {
         double a = WAY(w, i, 2), asp = (a + fabs(a)) * 0.5, asn = (a - fabs(a)) * 0.5;
         for (int r = 1; r <= 2; r++) {
           double f1 = FV(Fv, r, i, 1), f2 = FV(Fv, r, i, 2), f3 = FV(Fv, r, i, 3);
-          if (!nnd) WFL(w, r, i, 2) = asp * f1 + asn * f2;
+          if (weno) { /* :333-344 */
+            double urh = weno_right(f1, f2, f3, (f2 - f1) * (f2 - f1), (f3 - f2) * (f3 - f2));
+            WFL(w, r, i, 2) = asp * (0.5 * f1 + 0.5 * f2) + asn * urh;
+          } else if (!nnd) WFL(w, r, i, 2) = asp * f1 + asn * f2;
           else {
             double ulg = f1 + 0.5 * (f2 - f1);
             double urg = f2 - 0.5 * fminmd(f2 - f1, f3 - f2);
@@ -485,7 +530,10 @@ static void rhs_velocity(const orc_t *o, work_t *w, int iv, int jv) {
         double a = WAY(w, i, ny - 1), asp = (a + fabs(a)) * 0.5, asn = (a - fabs(a)) * 0.5;
         for (int r = 1; r <= 2; r++) {
           double f1 = FV(Fv, r, i, ny - 1), f2 = FV(Fv, r, i, ny - 2), f3 = FV(Fv, r, i, ny - 3);
-          if (!nnd) WFL(w, r, i, ny - 1) = asp * f2 + asn * f1;
+          if (weno) { /* :369-380 */
+            double ulh = weno_left(f3, f2, f1, (f2 - f3) * (f2 - f3), (f1 - f2) * (f1 - f2));
+            WFL(w, r, i, ny - 1) = asp * ulh + asn * (0.5 * f2 + 0.5 * f1);
+          } else if (!nnd) WFL(w, r, i, ny - 1) = asp * f2 + asn * f1;
           else {
             double ulg = f2 + 0.5 * fminmd(f1 - f2, f2 - f3);
             double urg = f1 - 0.5 * (f1 - f2);

@@ -62,6 +62,18 @@ def main(ref: pathlib.Path):
             table[kgln][name][k] = val
     (HERE / "gauleg_table.json").write_text(json.dumps(table, indent=1))
 
+    # 4b) the half-range Gauss-Hermite table GaussHermite
+    #     (src/GKUA_lisansudu.f90:364-436) as decimal strings
+    table, nv = {}, None
+    for ln in src[363:436]:
+        mk = re.search(r"NV\.EQ\.(\d+)", ln)
+        if mk:
+            nv = mk.group(1)
+            table[nv] = {"AGH": {}, "HGH": {}}
+        for name, k, val in re.findall(r"(AGH|HGH)\((\d+)\)=([0-9.Ee+-]+)", ln):
+            table[nv][name][k] = val
+    (HERE / "gauher_table.json").write_text(json.dumps(table, indent=1))
+
     # 5) fort.21: the constants line printed by init (src/sub_init.f90:62-64)
     txt = (ref / "fort.21").read_text().split()
     nums = [float(t) for t in txt if re.match(r"^-?[0-9.]+(E[+-]?\d+)?$", t)]

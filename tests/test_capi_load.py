@@ -45,6 +45,21 @@ def test_kernel_selector_checked_without_gpu():
         assert b"split_kernels" in lib.gsis_last_error()
 
 
+def test_kinetic_scheme_checked_without_gpu():
+    """Only the three schemes RHSFDVsub implements (src/GKUA_Residual.f90:
+    UP1 = 0, NND4A2 = 1, WENO3 = 3) pass validation; the NS-side codes 2, 4-6
+    have no kinetic flux there."""
+    g = load_pkg()
+    lib = g.capi.load_library()
+    case = g.cases.cavity_case(8, 8, kn=0.1)
+    pack = g.cases.ParamPack(case)
+    h = ctypes.c_void_p()
+    for bad in (2, 4, 5, 6, -1):
+        pack.p.scheme = bad
+        assert lib.gsis_create(ctypes.byref(pack.p), ctypes.byref(h)) == g.capi.GSIS_EUNSUPPORTED
+        assert b"scheme" in lib.gsis_last_error()
+
+
 def test_null_params_rejected_without_gpu():
     g = load_pkg()
     lib = g.capi.load_library()

@@ -66,10 +66,12 @@ def test_couette_parity():
     assert rel_l2(gpu.residual(), cpu.residual()) <= TOL
 
 
-def test_iteration_count_to_convergence():
+@pytest.mark.parametrize("scheme", [1, 3], ids=["nnd", "weno3"])
+def test_iteration_count_to_convergence(scheme):
     """Same number of outer iterations to the residual threshold."""
     g = load_pkg()
     case = g.cases.cavity_case(10, 24, kn=0.1, lid=0.15, dt=1.0)
+    case.const["scheme"] = scheme
     gpu, cpu = _pair(case)
     sg, hg = gpu.run(2000, tol=2e-5, show=10)
     sc, hc = cpu.run(2000, tol=2e-5, show=10)
@@ -178,6 +180,50 @@ def test_one_pass_kernels_agree():
     for k, s in zip(ks, runs):
         assert rel_l2(s.frd(), ref) <= 1e-12, k
         s.close()
+
+
+# WENO3 fluxes (Iflag_scheme = 3) in every kinetic kernel: a cavity whose
+# 19 columns make two strips of 8 and a partial one (or one of 16 and a
+# partial one), the periodic Couette channel with seams, and the shipped
+# cylinder (farfield faces, curved cells).
+@pytest.mark.parametrize("kind,kernel", [
+    ("cavity", 0), ("cavity", 3), ("cavity", 2), ("cavity", 1), ("couette", 0), ("cylinder", 0)],
+    ids=["cavity19_strip8", "cavity19_strip16", "cavity19_tiles", "cavity19_split", "couette20_strip8",
+         "cylinder40_auto"])
+def test_weno3_against_oracle(kind, kernel):
+    g = load_pkg()
+    if kind == "cavity":
+        case = g.cases.cavity_case(19, 20, kn=0.1, lid=0.2, dt=0.8)
+    elif kind == "couette":
+        case = g.cases.couette_case(ny_cells=32, nx_cells=20, nv=16, kn=0.1, uwall=0.1, dt=0.5)
+    else:
+        case = cylinder_case()
+    case.const["scheme"] = g.capi.SCHEME_WENO3
+    print("case:", case.name, "WENO3 kernel", kernel, flush=True)
+    gpu, cpu = _pair(case, split_kernels=kernel)
+    for k in range(2):
+        gpu.step(2)
+        cpu.step(2)
+        _compare(gpu, cpu, f"{case.name} WENO3 kernel={kernel} step {2 * (k + 1)}")
+    assert rel_l2(gpu.residual(), cpu.residual()) <= TOL
+    gpu.close()
+    cpu.close()
+
+
+@pytest.mark.parametrize("nv", [16, 32], ids=["gh8", "gh16"])
+def test_gauss_hermite_grid_parity(nv):
+    """Gauss-Hermite velocity grid (Quadrature_Mode = 1): the effective
+    weights w exp(v^2) go through the same moment, wall and residual kernels."""
+    g = load_pkg()
+    case = g.cases.cavity_case(12, nv, kn=0.1, lid=0.15, dt=0.5, grid="gh")
+    print("case:", case.name, "GH", nv, flush=True)
+    gpu, cpu = _pair(case)
+    gpu.step(10)
+    cpu.step(10)
+    _compare(gpu, cpu, f"{case.name} GH step 10")
+    assert rel_l2(gpu.residual(), cpu.residual()) <= TOL
+    gpu.close()
+    cpu.close()
 
 
 def test_residual_cadence_matches_oracle():

@@ -3,7 +3,10 @@ independently of how the oracle computes it:
 
 * LU-SGS sweep == (I + D^-1 U)(D - L)^-1 (vol * RHS), solved densely with numpy
   (GKUA_Residual.f90:34-109: forward recursion, non-recursive backward pass);
-* NND upwind flux is exact on linear data (RHSFDVsub, GKUA_Residual.f90:145-320);
+* NND and WENO3 upwind fluxes are exact on linear data (RHSFDVsub,
+  GKUA_Residual.f90:145-320);
+* WENO3 takes its ideal weights 1/3, 2/3 where the two smoothness indicators
+  agree (a zigzag), and the smooth side's stencil across a step (ENO);
 * the diffuse wall carries zero net mass flux (NWGaussLegendre + GKUA_wall_boundary);
 * a gas at rest at the wall temperature is a fixed point of the whole step;
 * the reduced Shakhov equilibrium keeps n, nu, energy and has heat flux (1-Pr) q
@@ -21,7 +24,7 @@ from helpers import load_oracle, load_pkg
 F32_CLAMP = float(np.float32(1e-30))
 
 
-def _case_periodic_box(n=10, nv=8, cfc=1e-14):
+def _case_periodic_box(n=10, nv=8, cfc=1e-14, scheme=1):
     g = load_pkg().cases
     s = np.linspace(0.0, 1.0, n + 1)
     xn, yn = np.meshgrid(s, s, indexing="ij")
@@ -32,6 +35,7 @@ def _case_periodic_box(n=10, nv=8, cfc=1e-14):
     v, w = g.sinh_velocity_grid(nv, 6.0)
     const = g.constants_from_kn(1.0, dt=0.5)
     const["cfc"] = cfc
+    const["scheme"] = scheme
     return g.Case("box", mesh, bcs, v, w, v.copy(), w.copy(), const, "rest")
 
 
@@ -83,8 +87,9 @@ def test_sweep_equals_dense_lusgs():
     assert worst < 1e-12, worst
 
 
-def test_nnd_flux_exact_on_linear_data():
-    case = _case_periodic_box()
+@pytest.mark.parametrize("scheme", [1, 3], ids=["nnd", "weno3"])
+def test_nnd_flux_exact_on_linear_data(scheme):
+    case = _case_periodic_box(scheme=scheme)
     o = load_oracle().OracleSolver(case)
     o.initialize()
     F = o.frd()
@@ -105,6 +110,131 @@ def test_nnd_flux_exact_on_linear_data():
             for r in range(2):
                 g0 = F[r, i + 3, j + 3] / lin[i + 3, j + 3]
                 want = -n * g0 * (VX * ex + VY * ey)
+                got = R[r, i + 3, j + 3]
+                worst = max(worst, np.max(np.abs(got - want)) / np.max(np.abs(want)))
+    assert worst < 1e-9, worst
+
+
+def _rhs_of(case, shape):
+    """Stored RHS of one step from the rest state multiplied cell by cell by
+    ``shape`` (indexed like F's spatial axes), and the unperturbed F."""
+    o = load_oracle().OracleSolver(case)
+    o.initialize()
+    g0 = o.frd()
+    o.upload(frd=g0 * shape[None, :, :, None, None])
+    o.step(1)
+    return o.frd00(), g0
+
+
+@pytest.mark.parametrize("scheme,gain", [(3, 4.0 / 3.0), (1, 2.0)], ids=["weno3", "nnd"])
+def test_weno3_ideal_weights_on_zigzag(scheme, gain):
+    """F = g0 (1 + zx (-1)^i + zy (-1)^j): the cell differences alternate in
+    sign with equal magnitude, so IS_1 = IS_2 = IS_3 and the WENO3 weights are
+    exactly the linear ones, 1/3 and 2/3.  The face states are then
+    g0 (1 -+ 2/3 z (-1)^i) and the RHS of cell (i, j) is
+    -n g0 (4/3) (zx (-1)^i |vx| + zy (-1)^j |vy|) on the n x n unit box.  The
+    minmod slopes of NND vanish on the same data, leaving first-order upwind
+    (gain 2); swapped linear weights would give 8/3."""
+    zx, zy = 1e-2, 3e-3
+    case = _case_periodic_box(scheme=scheme)
+    ii = np.arange(-3, case.nx + 4)[:, None]
+    jj = np.arange(-3, case.ny + 4)[None, :]
+    R, g0 = _rhs_of(case, 1.0 + zx * (-1.0) ** ii + zy * (-1.0) ** jj)
+    n = case.nx - 1
+    AX = np.abs(case.vi)[:, None]
+    AY = np.abs(case.vj)[None, :]
+    worst = 0.0
+    for i in range(3, case.nx - 2):
+        for j in range(3, case.ny - 2):
+            for r in range(2):
+                g = g0[r, i + 3, j + 3]
+                want = -n * gain * g * (zx * (-1.0) ** i * AX + zy * (-1.0) ** j * AY)
+                got = R[r, i + 3, j + 3]
+                worst = max(worst, np.max(np.abs(got - want)) / np.max(np.abs(want)))
+    assert worst < 1e-9, worst
+
+
+@pytest.mark.parametrize("axis", [0, 1], ids=["i", "j"])
+@pytest.mark.parametrize("where", ["box", "cavity_low", "cavity_high"])
+def test_weno3_step_takes_smooth_stencil(where, axis):
+    """F = g0 (1 + h [i >= k]), constant in j (and the same with i and j
+    exchanged): at every face the WENO3 weight of a stencil crossing the
+    jump is O((eps / IS)^2), so each state comes from the flat side and the
+    fluxes are first-order upwind: RHS(k-1) = -n min(vx,0)
+    g0 h, RHS(k) = -n max(vx,0) g0 h and zero one cell further out.  Pairing a
+    smoothness indicator with the wrong stencil puts g0 (1 + h/2) or
+    g0 (1 - h/2) on the jump instead.  In the cavity the jump sits at the
+    wall-adjacent face 3 or n-2, whose neighbour face (2 or n-1) mixes the
+    central and the WENO3 state."""
+    h = 0.5
+    if where == "box":
+        case = _case_periodic_box(scheme=3)
+        k = 5
+    else:
+        case = load_pkg().cases.cavity_case(10, 8, kn=0.1, lid=0.0, dt=0.5)
+        case.const["cfc"] = 1e-14
+        case.const["scheme"] = 3
+        k = 3 if where == "cavity_low" else case.nx - 2
+    ii = np.arange(-3, case.nx + 4)[:, None] + 0 * np.arange(-3, case.ny + 4)[None, :]
+    R, g0 = _rhs_of(case, np.where((ii if axis == 0 else ii.T) >= k, 1.0 + h, 1.0))
+    n = case.nx - 1
+    V = (case.vi[:, None] + 0 * case.vj[None, :]) if axis == 0 else (0 * case.vi[:, None] + case.vj[None, :])
+    worst = 0.0
+    for j in range(2, case.ny - 1):
+        for r in range(2):
+            for i, want in ((k - 2, 0 * V), (k - 1, -n * np.minimum(V, 0) * h), (k, -n * np.maximum(V, 0) * h),
+                            (k + 1, 0 * V)):
+                if not 2 <= i <= case.nx - 2:
+                    continue  # cells next to a wall face see the wall's own flux
+                c = (r, i + 3, j + 3) if axis == 0 else (r, j + 3, i + 3)
+                g = g0[c]
+                scale = n * np.max(np.abs(V) * g) * h
+                worst = max(worst, np.max(np.abs(R[c] - want * g)) / scale)
+    assert worst < 1e-6, worst
+
+
+def test_weno3_wall_adjacent_faces_on_zigzag():
+    """Next to a wall the WENO3 state on the wall side is the central average
+    (GKUA_Residual.f90:202-250, 333-380).  On the zigzag of the test above the
+    states at face f (s = (-1)^f, c the other direction's factor) are
+    bulk:  UL = g0 (c - 2/3 z s), UR = g0 (c + 2/3 z s);
+    f = 2:  UL = g0 c,             UR = g0 (c + 2/3 z s);
+    f = n-1: UL = g0 (c - 2/3 z s), UR = g0 c;
+    the cavity's cells 2..n-2 in both directions see every rule but the wall
+    face itself."""
+    zx, zy = 1e-2, 3e-3
+    g = load_pkg()
+    case = g.cases.cavity_case(10, 8, kn=0.1, lid=0.0, dt=0.5)
+    case.const["cfc"] = 1e-14
+    case.const["scheme"] = 3
+    nx, ny = case.nx, case.ny
+    ii = np.arange(-3, nx + 4)[:, None]
+    jj = np.arange(-3, ny + 4)[None, :]
+    R, g0 = _rhs_of(case, 1.0 + zx * (-1.0) ** ii + zy * (-1.0) ** jj)
+    n = nx - 1
+    VX = case.vi[:, None] + 0 * case.vj[None, :]
+    VY = 0 * case.vi[:, None] + case.vj[None, :]
+
+    def states(f, last, c, z):
+        s = (-1.0) ** f
+        ul = c if f == 2 else c - 2.0 / 3.0 * z * s
+        ur = c if f == last else c + 2.0 / 3.0 * z * s
+        return ul, ur
+
+    def flux(f, last, c, z, v):
+        ul, ur = states(f, last, c, z)
+        return np.maximum(v, 0) * ul + np.minimum(v, 0) * ur
+
+    worst = 0.0
+    for i in range(2, nx - 1):
+        for j in range(2, ny - 1):
+            cx = 1.0 + zy * (-1.0) ** j  # factor along a row
+            cy = 1.0 + zx * (-1.0) ** i
+            net = (flux(i + 1, nx - 1, cx, zx, VX) - flux(i, nx - 1, cx, zx, VX)
+                   + flux(j + 1, ny - 1, cy, zy, VY) - flux(j, ny - 1, cy, zy, VY))
+            for r in range(2):
+                gg = g0[r, i + 3, j + 3]
+                want = -n * gg * net
                 got = R[r, i + 3, j + 3]
                 worst = max(worst, np.max(np.abs(got - want)) / np.max(np.abs(want)))
     assert worst < 1e-9, worst

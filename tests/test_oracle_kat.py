@@ -2,6 +2,7 @@
 data recorded by the reference itself (tests/golden, written by
 make_fixtures.py from the reference's output files and sources)."""
 import json
+import math
 
 import numpy as np
 import pytest
@@ -24,6 +25,60 @@ def test_gauleg_table_is_float32_of_reference_literals():
         assert np.array_equal(np.array(xs), np.array(agl)), kgln
         for k, a in t["AGL"].items():
             assert ws[float(np.float32(float(a)))] == float(np.float32(float(t["HGL"][k]))), (kgln, k)
+
+
+def test_gauher_table_is_float32_of_reference_literals():
+    """GaussHermite's 8- and 16-point tables (src/GKUA_lisansudu.f90:384-433)
+    are REAL(4) literals; the float32 roundings of the half-range rule computed
+    from exp(-v^2) on [0, inf) must equal them.  The 7-point table
+    (:369-383) is not that rule: it integrates v^k exp(-v^2) exactly only up
+    to k = 6 (a 7-point Gauss rule reaches k = 13), so it cannot be derived
+    and the host-side producer refuses NV = 7."""
+    g = load_pkg()
+    tab = json.loads((GOLDEN / "gauher_table.json").read_text())
+    for nv in (8, 16):
+        t = tab[str(nv)]
+        x, w = g.cases.gauss_hermite_half_f32(nv)
+        agh = np.array([float(np.float32(float(t["AGH"][str(k + 1)]))) for k in range(nv)])
+        hgh = np.array([float(np.float32(float(t["HGH"][str(k + 1)]))) for k in range(nv)])
+        assert np.array_equal(x, agh), nv
+        assert np.array_equal(w, hgh), nv
+    a7 = np.array([float(tab["7"]["AGH"][str(k + 1)]) for k in range(7)])
+    h7 = np.array([float(tab["7"]["HGH"][str(k + 1)]) for k in range(7)])
+    mom = [abs(np.sum(h7 * a7 ** k) / (math.gamma((k + 1) / 2) / 2) - 1.0) for k in range(14)]
+    assert max(mom[:7]) < 1e-8 and min(mom[7:]) > 1e-6, mom
+    with pytest.raises(ValueError):
+        g.cases.gauss_hermite_half_f32(7)
+
+
+def test_gauss_hermite_grid_integrates_polynomials():
+    """The mirrored 2*NV-point grid with w_eff = w exp(v^2) integrates
+    v^k exp(-v^2) over the real line exactly up to degree 2 NV - 1 (the
+    half-range Gauss rule on each side), here to the float32 rounding of its
+    table."""
+    g = load_pkg()
+    for nv in (8, 16):
+        v, w = g.cases.gh_velocity_grid(nv)
+        assert np.all(np.diff(v) > 0) and len(v) == 2 * nv
+        for k in range(2 * nv):
+            got = np.sum(w * np.exp(-v * v) * v ** k)
+            want = 0.0 if k % 2 else math.gamma((k + 1) / 2)
+            assert abs(got - want) <= 2e-7 * math.gamma((k + 1) / 2 + 0.5), (nv, k, got, want)
+
+
+def test_gauss_hermite_input_read_from_quadrature_para(tmp_path):
+    """Quadrature_Para.dat with method 1 and option 2 selects the 8-point
+    half-range rule in both directions (src/opencfd_ec2d.f90:84-98)."""
+    g = load_pkg()
+    src = GOLDEN / "cylinder"
+    for f in ("generic.dat", "generic.inp", "control.in"):
+        (tmp_path / f).write_text((src / f).read_text())
+    lines = (src / "Quadrature_Para.dat").read_text().splitlines()
+    lines[2] = "1"
+    (tmp_path / "Quadrature_Para.dat").write_text("\n".join(lines) + "\n")
+    case = g.cases.shipped_case(tmp_path, "cylinder40-gh")
+    v, w = g.cases.gh_velocity_grid(8)
+    assert np.array_equal(case.vi, v) and np.array_equal(case.wj, w)
 
 
 def test_rcon_matches_fort21():
