@@ -9,6 +9,6 @@ Submodules:
   fields  -- macroscopic fields and their relative L2 differences
   build   -- hipcc build of libgsis.so for gfx950 (and the flang host side)
 """
-from . import capi, cases, fields  # noqa: F401
+from . import capi, cases, fields, restart  # noqa: F401
 
 __all__ = ["capi", "cases", "fields"]

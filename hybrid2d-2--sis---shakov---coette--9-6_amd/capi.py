@@ -52,6 +52,7 @@ EXPORTED = (
     "gsis_download_macro",
     "gsis_download_frd",
     "gsis_counters",
+    "gsis_set_counters",
     "gsis_timing",
     "gsis_destroy",
     "gsis_last_error",
@@ -153,6 +154,7 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.gsis_download_macro.argtypes = [h] + [c_dp] * 9
     lib.gsis_download_frd.argtypes = [h, c_dp]
     lib.gsis_counters.argtypes = [h, ctypes.POINTER(ctypes.c_int32), c_dp, ctypes.POINTER(ctypes.c_int32)]
+    lib.gsis_set_counters.argtypes = [h, ctypes.c_int32, ctypes.c_double]
     lib.gsis_timing.argtypes = [h, ctypes.c_int32, c_dp, ctypes.POINTER(ctypes.c_int32)]
     lib.gsis_destroy.argtypes = [h]
     lib.gsis_last_error.restype = ctypes.c_char_p

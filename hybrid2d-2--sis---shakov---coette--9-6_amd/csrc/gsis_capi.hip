@@ -588,4 +588,13 @@ int gsis_counters(gsis_handle *h, int32_t *kstep, double *tt, int32_t *ns_iter) 
   return 0;
 }
 
+int gsis_set_counters(gsis_handle *h, int32_t kstep, double tt) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  if (kstep < 0) return fail(GSIS_EARG, "kstep must be >= 0");
+  // the counters are read when a step is queued; no wait on the stream
+  h->kstep = kstep;
+  h->tt = tt;
+  return 0;
+}
+
 }  // extern "C"

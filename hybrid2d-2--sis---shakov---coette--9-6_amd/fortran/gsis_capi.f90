@@ -92,6 +92,12 @@ module gsis_capi
       integer(c_int32_t), intent(out) :: kstep, ns_iter
       real(c_double), intent(out) :: tt
     end function
+    integer(c_int) function gsis_set_counters(h, kstep, tt) bind(C, name="gsis_set_counters")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: kstep
+      real(c_double), value :: tt
+    end function
     integer(c_int) function gsis_destroy(h) bind(C, name="gsis_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: h

@@ -9,9 +9,13 @@
 ! Newton-Cotes weights times dv/ds; Kn given, cfc = sqrt(pi)/(2 Kn), Pr=2/3,
 ! omega=0.81, Shakhov model, NND flux.
 !
-! usage: gsis_cavity_host n nv kn lid dt nsteps kstep_show
+! usage: gsis_cavity_host n nv kn lid dt nsteps kstep_show [restart]
 ! prints "res <kstep> r1 r2 r3 r4" every kstep_show steps and finally
 ! "sum <sum of S_GKUA(1:4) over cells 0..n> <sum of CF>".
+! restart = save1 / save2: after the run, back up into restart1 / restart2
+! (backup_Macro_write, src/Backup_write_read.f90:114-209; the directories
+! restart?/star00 must exist, as for the reference); restart = resume: before
+! the run, restore from the backups as init_flow_read does (:4-112).
 program gsis_cavity_host
   use iso_c_binding
   use gsis_capi
@@ -19,7 +23,7 @@ program gsis_cavity_host
   integer :: n, nv, nsteps, nshow, nx, ny, i, j, k, iv, jv, done
   real(c_double) :: kn, lid, dt, hs, p0, g, ma, cfc, sres(4), tt
   integer(c_int32_t) :: kstep, nsit
-  character(len=64) :: arg
+  character(len=64) :: arg, rmode
   real(c_double), allocatable, target :: x(:,:), y(:,:), vol(:,:), si(:,:), sj(:,:), ni1(:,:), ni2(:,:), &
                                          nj1(:,:), nj2(:,:), vi(:), wi(:), frd(:,:,:,:,:), s(:,:,:), &
                                          cf(:,:), u(:,:,:)
@@ -35,6 +39,8 @@ program gsis_cavity_host
   call get_command_argument(5, arg); read(arg, *) dt
   call get_command_argument(6, arg); read(arg, *) nsteps
   call get_command_argument(7, arg); read(arg, *) nshow
+  rmode = 'none'
+  if (command_argument_count() >= 8) call get_command_argument(8, rmode)
   nx = n + 1
   ny = n + 1
 
@@ -103,6 +109,7 @@ program gsis_cavity_host
   call gsis_check(gsis_upload_frd(h, frd), 'gsis_upload_frd')
   call gsis_check(gsis_upload_macro(h, c_loc(s), c_loc(cf), c_loc(u), c_null_ptr), 'gsis_upload_macro')
   call gsis_check(gsis_refresh_moments(h), 'gsis_refresh_moments')
+  if (rmode == 'resume') call read_backup()
 
   ! main loop
   done = 0
@@ -117,9 +124,66 @@ program gsis_cavity_host
   call gsis_check(gsis_download_macro(h, c_loc(s), c_null_ptr, c_loc(cf), c_null_ptr, c_null_ptr, &
                                       c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), 'gsis_download_macro')
   write(*, '(a,2(1x,es24.16))') 'sum', sum(s(1:4, 0:nx, 0:ny)), sum(cf(0:nx, 0:ny))
+  if (rmode == 'save1') call write_backup(1)
+  if (rmode == 'save2') call write_backup(2)
   call gsis_check(gsis_destroy(h), 'gsis_destroy')
 
 contains
+
+  ! backup_Macro_write(flag) with backup_Mesco_write: B%FRD as one unformatted
+  ! record, then flag.dat = 1, tt, Kstep (one rank: star00/restart00000.dat)
+  subroutine write_backup(flag)
+    integer, intent(in) :: flag
+    character(len=64) :: fname
+    integer :: un
+    call gsis_check(gsis_download_frd(h, frd), 'gsis_download_frd')
+    call gsis_check(gsis_counters(h, kstep, tt, nsit), 'gsis_counters')
+    write(fname, '(a,i1,a)') 'restart', flag, '/star00/restart00000.dat'
+    open(newunit=un, file=trim(fname), form='unformatted', status='replace')
+    write(un) frd
+    close(un)
+    write(fname, '(a,i1,a)') 'restart', flag, '/flag.dat'
+    open(newunit=un, file=trim(fname), status='replace')
+    write(un, *) 1
+    write(un, *) tt
+    write(un, *) kstep
+    close(un)
+  end subroutine write_backup
+
+  ! init_flow_read + backup_Mesco_read: restart2 when its flag is 1, else
+  ! restart1; then the kinetic BCs and moments of the restored distribution
+  ! and the saved counters
+  subroutine read_backup()
+    character(len=64) :: fname
+    integer :: un, flag, slot, ios
+    integer(c_int32_t) :: ks
+    real(c_double) :: t
+    slot = 0
+    do k = 2, 1, -1
+      write(fname, '(a,i1,a)') 'restart', k, '/flag.dat'
+      open(newunit=un, file=trim(fname), action='read', status='old', iostat=ios)
+      if (ios /= 0) cycle
+      read(un, *) flag
+      read(un, *) t
+      read(un, *) ks
+      close(un)
+      if (flag == 1) then
+        slot = k
+        exit
+      end if
+    end do
+    if (slot == 0) then
+      write(*, '(a)') 'No backup data is available'
+      stop 1
+    end if
+    write(fname, '(a,i1,a)') 'restart', slot, '/star00/restart00000.dat'
+    open(newunit=un, file=trim(fname), form='unformatted', action='read', status='old')
+    read(un) frd
+    close(un)
+    call gsis_check(gsis_upload_frd(h, frd), 'gsis_upload_frd')
+    call gsis_check(gsis_refresh_moments(h), 'gsis_refresh_moments')
+    call gsis_check(gsis_set_counters(h, ks, t), 'gsis_set_counters')
+  end subroutine read_backup
 
   ! ghost nodes of the physical boundaries by linear extrapolation, in BC
   ! order, then the corners (src/sub_init.f90:777-810)

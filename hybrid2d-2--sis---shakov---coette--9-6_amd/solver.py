@@ -7,7 +7,7 @@ import ctypes
 
 import numpy as np
 
-from . import capi
+from . import capi, restart
 from .cases import Case, ParamPack, frd_shape, initial_state, ns_boundary_init, swall_len
 
 
@@ -73,6 +73,24 @@ class GsisSolver:
         a = np.zeros(frd_shape(self.case), order="F")
         capi.check(self.lib, self.lib.gsis_download_frd(self.h, _dp(a)))
         return a
+
+    # -- restart (src/Backup_write_read.f90) -------------------------------
+    def save_restart(self, root, flag: int):
+        """backup_Macro_write(flag): the distribution and flag.dat under
+        ``root``/restart{flag}."""
+        k, tt, _ = self.counters()
+        return restart.write_restart(root, flag, self.frd(), tt, k)
+
+    def resume(self, root) -> int:
+        """init_flow_read: the distribution of the newest valid backup slot,
+        the kinetic BCs and moments of it, and the saved step counter.  The
+        macroscopic state (S, CF, U) is whatever the handle holds, as in the
+        reference, where only FRD is saved.  Returns the slot read."""
+        frd, tt, kstep, slot = restart.read_restart(root, frd_shape(self.case))
+        self.upload(frd=frd)
+        self.refresh_moments()
+        capi.check(self.lib, self.lib.gsis_set_counters(self.h, int(kstep), float(tt)))
+        return slot
 
     # -- stepping ---------------------------------------------------------
     def step(self, n: int = 1):

@@ -161,6 +161,13 @@ int gsis_download_frd(gsis_handle *h, double *frd);
  * (MP%iter). */
 int gsis_counters(gsis_handle *h, int32_t *kstep, double *tt, int32_t *ns_iter);
 
+/* Restore the step counter and accumulated time that init_flow_read takes
+ * from restart{1,2}/flag.dat (src/Backup_write_read.f90:20-23,29-32), after
+ * gsis_upload_frd and gsis_refresh_moments have restored the distribution
+ * (backup_Mesco_read, :93-109).  The residual cadence (kstep_show) follows the
+ * restored count.  kstep must be >= 0. */
+int gsis_set_counters(gsis_handle *h, int32_t kstep, double tt);
+
 /* Kernel timing with HIP events on the handle's stream.  No counterpart in
  * the reference: it exists for benchmarks.  enable=1 clears and starts the
  * accumulators, 0 clears and stops them, -1 only reads.  ms[0..2] receive the

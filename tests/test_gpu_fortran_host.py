@@ -1,6 +1,7 @@
 """The Fortran host program (fortran/gsis_cavity_host.f90), which drives the
 library through the ISO_C_BINDING module the way Hybrid2D's main loop would,
-reproduces the oracle's residual history and moments on a lid-driven cavity."""
+reproduces the oracle's residual history and moments on a lid-driven cavity,
+and backs up / resumes through the reference's restart files."""
 import pathlib
 import subprocess
 
@@ -15,7 +16,69 @@ HOST = pathlib.Path(__file__).resolve().parents[1] / "hybrid2d-2--sis---shakov--
 TOL = 1e-10
 
 
-@pytest.mark.skipif(not HOST.exists(), reason="Fortran host not built (flang absent at build time)")
+def _run_host(args, cwd=None):
+    out = subprocess.run([str(HOST), *map(str, args)], capture_output=True, text=True, timeout=120, cwd=cwd)
+    assert out.returncode == 0, out.stdout + out.stderr
+    res = {int(l.split()[1]): list(map(float, l.split()[2:])) for l in out.stdout.splitlines() if l.startswith("res")}
+    ssum = [list(map(float, l.split()[1:])) for l in out.stdout.splitlines() if l.startswith("sum")][0]
+    return res, ssum
+
+
+needs_host = pytest.mark.skipif(not HOST.exists(), reason="Fortran host not built (flang absent at build time)")
+
+
+@needs_host
+def test_fortran_backup_and_resume(tmp_path):
+    """20 steps in one run equal 10 steps, a backup into restart1 by the
+    Fortran host's own unformatted write, and 10 more steps in a new process
+    resumed from it (init_flow_read order).  The backup holds the oracle's
+    distribution after 10 steps and the step count, as read by the Python
+    reader of the same files."""
+    n, nv, kn, lid, dt, show = 12, 24, 0.1, 0.15, 0.5, 10
+    for d in ("restart1/star00", "restart2/star00"):
+        (tmp_path / d).mkdir(parents=True)
+    print("case: Fortran host backup/resume cavity", n, nv, flush=True)
+    full, full_sum = _run_host([n, nv, kn, lid, dt, 20, show])
+    first, _ = _run_host([n, nv, kn, lid, dt, 10, show, "save1"], cwd=tmp_path)
+    g = load_pkg()
+    case = g.cases.cavity_case(n, nv, kn=kn, lid=lid, dt=dt)
+    frd, tt, kstep, slot = g.restart.read_restart(tmp_path, g.cases.frd_shape(case))
+    assert (slot, kstep) == (1, 10) and abs(tt - 10 * dt) < 1e-12
+    orc = load_oracle().OracleSolver(case)
+    orc.initialize()
+    orc.step(10)
+    assert np.linalg.norm(frd - orc.frd()) <= TOL * np.linalg.norm(orc.frd())
+    orc.close()
+    second, second_sum = _run_host([n, nv, kn, lid, dt, 10, show, "resume"], cwd=tmp_path)
+    print("full", full[20], full_sum, "resumed", second, second_sum, flush=True)
+    assert sorted(second) == [20]
+    assert first[10] == full[10]
+    assert np.linalg.norm(np.subtract(second[20], full[20])) <= 1e-12 * np.linalg.norm(full[20])
+    assert np.allclose(second_sum, full_sum, rtol=1e-13, atol=0)
+
+
+@needs_host
+def test_fortran_resumes_from_python_backup(tmp_path):
+    """A backup written by the Python writer (the oracle's distribution after
+    10 steps, in restart2) is read by the Fortran host's unformatted read, and
+    the resumed run matches the oracle's 20-step residual."""
+    n, nv, kn, lid, dt, show = 12, 24, 0.1, 0.15, 0.5, 10
+    g = load_pkg()
+    case = g.cases.cavity_case(n, nv, kn=kn, lid=lid, dt=dt)
+    orc = load_oracle().OracleSolver(case)
+    orc.initialize()
+    orc.step(10)
+    g.restart.write_restart(tmp_path, 2, orc.frd(), 10 * dt, 10)
+    orc.step(10)
+    want = orc.residual()
+    orc.close()
+    print("case: Fortran host resume from Python backup", n, nv, flush=True)
+    res, _ = _run_host([n, nv, kn, lid, dt, 10, show, "resume"], cwd=tmp_path)
+    assert sorted(res) == [20]
+    assert np.linalg.norm(np.subtract(res[20], want)) <= TOL * np.linalg.norm(want)
+
+
+@needs_host
 def test_fortran_host_matches_oracle():
     n, nv, kn, lid, dt, steps, show = 12, 24, 0.1, 0.15, 0.5, 20, 10
     print("case: Fortran host cavity", n, nv, kn, lid, dt, steps, show, flush=True)

@@ -226,6 +226,36 @@ def test_gauss_hermite_grid_parity(nv):
     cpu.close()
 
 
+def test_backup_and_resume_continue_the_run(tmp_path):
+    """Python host: 16 steps equal 8 steps, save_restart into restart2, and 8
+    steps of a new handle resumed from it (distribution, BCs + moments, step
+    counter and time)."""
+    g = load_pkg()
+    case = g.cases.cavity_case(14, 20, kn=0.1, lid=0.2, dt=0.7)
+    print("case:", case.name, "backup/resume", flush=True)
+    full = solver_mod().GsisSolver(case, kstep_show=4)
+    full.initialize()
+    full.step(16)
+    a = solver_mod().GsisSolver(case, kstep_show=4)
+    a.initialize()
+    a.step(8)
+    a.save_restart(tmp_path, 2)
+    a.close()
+    b = solver_mod().GsisSolver(case, kstep_show=4)
+    b.initialize()
+    assert b.resume(tmp_path) == 2
+    assert b.counters()[0] == 8
+    b.step(8)
+    assert b.counters()[:2] == full.counters()[:2]
+    assert rel_l2(b.frd(), full.frd()) <= 1e-12
+    assert rel_l2(b.residual(), full.residual()) <= 1e-12
+    mb, mf = b.macro(), full.macro()
+    for k in ("s", "cf"):
+        assert rel_l2(mb[k], mf[k]) <= 1e-12, k
+    b.close()
+    full.close()
+
+
 def test_residual_cadence_matches_oracle():
     """With kstep_show=10 the RHS is kept only every 10th step; the residual
     history and the iteration count still match the oracle's."""
