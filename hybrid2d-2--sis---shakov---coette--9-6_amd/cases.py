@@ -353,6 +353,113 @@ def read_quadrature_para(path: str | pathlib.Path) -> dict:
 
 
 # ---------------------------------------------------------------------------
+# writers of the same files, so that a case built here can be given to the
+# reference as input (the readers above read them back unchanged)
+# ---------------------------------------------------------------------------
+def _f(v: float) -> str:
+    return repr(float(v))
+
+
+def write_generic_dat(path: str | pathlib.Path, xn: np.ndarray, yn: np.ndarray) -> None:
+    """generic.dat of one block: NB; NI NJ NK=1; x, y, z column-major,
+    list-directed (src/sub_init.f90:102-178 reads x and y)."""
+    ni, nj = xn.shape
+    vals = np.concatenate([np.asarray(xn, float).ravel(order="F"), np.asarray(yn, float).ravel(order="F"),
+                           np.zeros(ni * nj)])
+    rows = [" ".join(f"{v:24.16E}" for v in vals[k:k + 4]) for k in range(0, len(vals), 4)]
+    pathlib.Path(path).write_text(f"1\n{ni} {nj} 1\n" + "\n".join(rows) + "\n")
+
+
+def write_generic_inp(path: str | pathlib.Path, nx: int, ny: int, bcs: Sequence[BC]) -> None:
+    """generic.inp of one GKUA block (solver 2): the subfaces with their
+    positive codes in list order (src/sub_init.f90:236-284).  The format has no
+    wall velocity (the reference sets it to 0, src/GKUA_boundary.f90:66-67), so
+    a moving wall is refused."""
+    lines = [" generic.inp", "1", f"{nx} {ny}", "2", str(len(bcs))]
+    for b in bcs:
+        if b.uw != 0.0 or b.vw != 0.0:
+            raise ValueError("generic.inp cannot express a moving wall")
+        lines.append(f"{b.ist:8d} {b.iend:8d} {b.jst:8d} {b.jend:8d} {-b.code:8d}")
+    pathlib.Path(path).write_text("\n".join(lines) + "\n")
+
+
+def write_control_in(path: str | pathlib.Path, c: dict) -> None:
+    """control.in with the keys read_control_in returns, on the lines
+    read_parameter reads (src/opencfd_ec2d.f90:123-141)."""
+    v = {k: _f(val) for k, val in c.items()}
+    i = {k: str(int(c[k])) for k in ("KHS", "If_viscous", "Time_Method", "Iflag_Scheme", "Iflag_Flux",
+                                     "IFlag_Reconstruction", "Iflag_init", "Iflag_local_dt", "Kstep_save",
+                                     "Kstep_show")}
+    text = [
+        " #Inflow Condition#",
+        " Mole_Mass,   gamma,  KHS,",
+        f"{v['Mole_Mass']} {v['gamma']} {i['KHS']}",
+        "VHS model :        Alpha       Omiga        Vis_ref            Tref",
+        f"{v['ALPgas']} {v['OMGgas']} {v['vis_ref']} {v['TREF']}",
+        "Sutherland model : S_k        Vis_ref       Tref",
+        f"{v['Sgas']} {v['vis_ref']} {v['TREF']}",
+        "LJ 12-6 model :    DSLJ,     TSLJ,      vis_ref                Tref",
+        f"{v['DSLJ']} {v['TSLJ']} {v['vis_ref']} {v['TREF']}",
+        " Ma           ND_inf(1)     T_inf(K)      Twall(K)    P_outlet      AoA(degree)        Lref(m)",
+        f"{v['Ma']} {v['ND_inf']} {v['T_inf']} {v['Twall']} {v['P_outlet']} {v['AoA']} {v['Lref']}",
+        "#Numerical Scheme for NS#",
+        "Iflag_viscous    Time_Method   Iflag_scheme  Iflag_flux  IFlag_Reconstruction",
+        f"{i['If_viscous']} {i['Time_Method']} {i['Iflag_Scheme']} {i['Iflag_Flux']} {i['IFlag_Reconstruction']}",
+        "#Running Model Control#",
+        " Iflag_init       Iflag_local_dt,     dt_global,         CFL,       dtmax,       dtmin,     tend"
+        "        Kstep_save        Kstep_show",
+        f"{i['Iflag_init']} {i['Iflag_local_dt']} {v['dt_global']} {v['CFL']} {v['dtmax']} {v['dtmin']} "
+        f"{v['t_end']} {i['Kstep_save']} {i['Kstep_show']}",
+    ]
+    pathlib.Path(path).write_text("\n".join(text) + "\n")
+
+
+def write_quadrature_para(path: str | pathlib.Path, q: dict) -> None:
+    """Quadrature_Para.dat (src/opencfd_ec2d.f90:79-121) for method 1
+    (Gauss-Hermite, ``ngh`` = 7, 8 or 16) or 2 (Gauss-Legendre, ``dvx, nvi,
+    vxdown, vxup, dvy, nvj, vydown, vyup, kgln``); the other method's lines
+    hold placeholders."""
+    gh = {7: 1, 8: 2, 16: 3}[q.get("ngh", 8)]
+    gl = q if q["mode"] == 2 else dict(dvx=1.2, nvi=15, vxdown=-9.0, vxup=9.0, dvy=1.2, nvj=15, vydown=-9.0,
+                                      vyup=9.0, kgln=4)
+    text = [
+        "##Quadrature method & parameters specify",
+        "Quadrature method:1:--Gauss Hermite ;2:--Gauss Legendre;3:--Newton-Cotes",
+        str(int(q["mode"])),
+        "#parameter for various quadrature method.",
+        "###############Gauss Hermite#################",
+        "#.1:--7 GH points;2:--8 GH points; 3--16 GH points",
+        str(gh),
+        "###############Gauss Legendre ################",
+        "#Dvx    Intervals  Vx-lower-bound  Vx upper bound",
+        f"{_f(gl['dvx'])} {int(gl['nvi'])} {_f(gl['vxdown'])} {_f(gl['vxup'])}",
+        "#Dvy    Intervals  Vy-lower-bound  Vy upper bound",
+        f"{_f(gl['dvy'])} {int(gl['nvj'])} {_f(gl['vydown'])} {_f(gl['vyup'])}",
+        "#Inner division:KGLN  --3,4,5,6,9,10 are available",
+        str(int(gl["kgln"])),
+    ]
+    pathlib.Path(path).write_text("\n".join(text) + "\n")
+
+
+def control_for_kn(kn: float, omega: float = 0.81, gamma: float = 5.0 / 3.0, ma: float = 1.0, dt: float = 0.01,
+                   t_inf: float = 273.15, mole_mass: float = 40.0, vis_ref: float = 2.117e-5,
+                   scheme: int = capi.SCHEME_NND, kstep_show: int = 50) -> dict:
+    """A control.in (VHS model) whose constants, as init derives them
+    (constants_from_control), are those of constants_from_kn(kn): ND_inf is
+    solved from Re = cfc Ma sqrt(2 gamma) with cfc = sqrt(pi)/(2 Kn).  The
+    Prandtl number follows from gamma in init (2/3 for gamma = 5/3)."""
+    rcon = BOLTZ_F32 / (MATOM_F32 * mole_mass)
+    re = math.sqrt(math.pi) / (2.0 * kn) * ma * math.sqrt(2.0 * gamma)
+    vis_inf = vis_ref  # T_inf = TREF
+    nd = re * vis_inf / (MATOM_F32 * mole_mass * ma * math.sqrt(gamma * rcon * t_inf) * 1.0)
+    return dict(Mole_Mass=mole_mass, gamma=gamma, KHS=0, ALPgas=1.0, OMGgas=omega, vis_ref=vis_ref, TREF=t_inf,
+                Sgas=148.0, DSLJ=3.798e-10, TSLJ=71.4, Ma=ma, ND_inf=nd, T_inf=t_inf, Twall=t_inf, P_outlet=-1.0,
+                AoA=0.0, Lref=1.0, If_viscous=1, Time_Method=0, Iflag_Scheme=scheme, Iflag_Flux=5,
+                IFlag_Reconstruction=0, Iflag_init=0, Iflag_local_dt=1, dt_global=dt, CFL=20.0, dtmax=10.0,
+                dtmin=1e-7, t_end=1000.0, Kstep_save=100, Kstep_show=kstep_show)
+
+
+# ---------------------------------------------------------------------------
 # constants of init (src/sub_init.f90:6-79)
 # ---------------------------------------------------------------------------
 BOLTZ_F32 = float(F32(1.38065e-23))  # Boltz, REAL(4) literal (src/COMMONDATA.f90:9)
