@@ -531,14 +531,15 @@ __global__ void __launch_bounds__(256) k_sweep(Dev d, int VT, T *FG, T *FH, cons
 constexpr int kNW = kNCellPar;
 // With pstride = 0 the constants are stored by cell instead, cpar[((j-1)*(nx-1) + i-1)*kNW + k]
 // (the layout of the column-wavefront kernel).
-__global__ void k_cellpar(Dev d, double *cpar, int pstride) {
+template <typename P>
+__global__ void k_cellpar(Dev d, P *cpar, int pstride) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ncol = d.nx - 1, nrow = d.ny - 1;
   if (t >= ncol * nrow) return;
   const int i = t % ncol + 1, j = t / ncol + 1;
   const int c = cidx(d, i, j), N = d.ncell;
   const int r = (i + j + 2) % (nrow + 4);
-  double *o = pstride ? cpar + (size_t)r * pstride + (size_t)(i - 1) * kNW : cpar + (size_t)t * kNW;
+  P *o = pstride ? cpar + (size_t)r * pstride + (size_t)(i - 1) * kNW : cpar + (size_t)t * kNW;
   const double n = d.S[c], T = d.S[3 * N + c];
   const int g0 = gidx(d, i, j), gi = gidx(d, i + 1, j), gj = gidx(d, i, j + 1);
   const double vol = d.vol[g0], cf = d.CF[c];
@@ -547,7 +548,7 @@ __global__ void k_cellpar(Dev d, double *cpar, int pstride) {
                             d.ni1[g0] * d.si[g0], d.ni2[g0] * d.si[g0], d.ni1[gi] * d.si[gi], d.ni2[gi] * d.si[gi],
                             d.nj1[g0] * d.sj[g0], d.nj2[g0] * d.sj[g0], d.nj1[gj] * d.sj[gj], d.nj2[gj] * d.sj[gj]};
 #pragma unroll
-  for (int k = 0; k < kNW; k++) o[k] = vals[k];
+  for (int k = 0; k < kNW; k++) o[k] = P(vals[k]);
 }
 
 // Flux through one face in the fused kernel: the rules of iface_flux /
@@ -1027,7 +1028,7 @@ __device__ __forceinline__ void lds_copy16(const void *src, void *dst) {
 // 4 wavefronts per SIMD (at most 128 VGPRs): with 80 KB of LDS per 8-column
 // workgroup two workgroups share a CU, which the register file must allow.
 template <typename T, int NW, int SL>
-__global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const double *__restrict__ cpar, T *__restrict__ F,
+__global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict__ cpar, T *__restrict__ F,
                                                  const T *__restrict__ slot, T *__restrict__ R, int store_rhs,
                                                  int ntiles, T *msg, T *dright, int *flag, int epoch, int *werr,
                                                  const int *__restrict__ oflags) {
@@ -1102,8 +1103,9 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const double *__res
   T mfg = T(0), mdg = T(0), mfh = T(0), mdh = T(0);  // seam message of row r+1
   int mflag = 0;                                     // flag of row r+2
   long long oc = -2 * rb + (long long)(i0 + 1) * cb;  // byte offset of cell (i, r), r = -3
-  long long ocp = ((long long)(-4) * ncol + (i0 - 1)) * (kNW * 8);  // cpar byte offset of cell (i, r)
-  const long long ccstep = (long long)ncol * (kNW * 8);
+  constexpr long long kRec = kNW * (long long)sizeof(T);        // bytes of one cell's constants
+  long long ocp = ((long long)(-4) * ncol + (i0 - 1)) * kRec;  // cpar byte offset of cell (i, r)
+  const long long ccstep = (long long)ncol * kRec;
   const size_t sbc = ((size_t)(g0w > 0 ? g0w - 1 : 0) * ntiles + tile) * (size_t)nrow;  // seam rows read (row r at sbc + r - 1)
   const size_t sbp = ((size_t)g0w * ntiles + tile) * (size_t)nrow;                    // seam rows written
   {
@@ -1147,8 +1149,8 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const double *__res
         const int *pc = reinterpret_cast<const int *>(reinterpret_cast<const char *>(cpar) + ocp + ccstep);
         pf0 = pc[0];
         pf1 = pc[16];
-        pf2 = pc[32];
-        pf3 = pc[kNW * 2 - 1];
+        pf2 = pc[kRec > 128 ? 32 : 0];
+        pf3 = pc[kRec / 4 - 1];
       }
       const bool real = r >= 1 && r <= nrow;
       const int rfl = real ? oflags[r] : 0;  // i-face overrides of row r
@@ -1170,7 +1172,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const double *__res
       T feqg = T(0), feqh = T(0), axl = T(0), axr = T(0), ayb = T(0), ayt = T(0), ra = T(0);
       T vinv = T(0), cf = T(0), vol = T(0);
       if (real) {
-        const double *cc = reinterpret_cast<const double *>(reinterpret_cast<const char *>(cpar) + ocp);
+        const T *cc = reinterpret_cast<const T *>(reinterpret_cast<const char *>(cpar) + ocp);
         {  // Shakhov equilibrium of cell (i,r) (Get_Feq, src/GKUA_Residual.f90:420-469)
           const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
           const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
@@ -1181,7 +1183,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const double *__res
           feqg = gd * ((T(1) - T(2) * fvx) + b);
           feqh = ((T(cc[3]) * gd) * T(0.5)) * (T(1) + b);
         }
-        const double *cc2 = cc;
+        const T *cc2 = cc;
         axl = vx * T(cc2[12]) + vy * T(cc2[13]);
         axr = vx * T(cc2[14]) + vy * T(cc2[15]);
         ayb = vx * T(cc2[16]) + vy * T(cc2[17]);
@@ -1338,7 +1340,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const double *__res
 // -A^-_x/alfa, -A^-_y/alfa recomputed from the cell constants as k_strip
 // computes them.  One thread per (seam, tile, row, velocity lane).
 template <typename T>
-__global__ void __launch_bounds__(256) k_seam(Dev d, const double *__restrict__ cpar, T *F, const T *msg,
+__global__ void __launch_bounds__(256) k_seam(Dev d, const T *__restrict__ cpar, T *F, const T *msg,
                                               const T *dright, int ntiles, int nw) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = (int)(t & 63);
@@ -1358,7 +1360,7 @@ __global__ void __launch_bounds__(256) k_seam(Dev d, const double *__restrict__ 
   const T gdn = r < nrow ? msg[(row_id + 1) * 256 + 64 + lane] : T(0);  // Delta*(i, j+1), 0 past the last row
   const T hdn = r < nrow ? msg[(row_id + 1) * 256 + 192 + lane] : T(0);
   const T gdr = dright[row_id * 128 + lane], hdr = dright[row_id * 128 + 64 + lane];
-  const double *cc = cpar + ((size_t)(r - 1) * ncol + (i - 1)) * kNW;
+  const T *cc = cpar + ((size_t)(r - 1) * ncol + (i - 1)) * kNW;
   const T axl = vx * T(cc[12]) + vy * T(cc[13]);
   const T axr = vx * T(cc[14]) + vy * T(cc[15]);
   const T ayb = vx * T(cc[16]) + vy * T(cc[17]);
@@ -1821,21 +1823,24 @@ bool launch_strip_nw(const Dev &d, double *cpar, T *F, const T *slot, T *R, int 
   const size_t rows = (size_t)(ngroups - 1) * ntiles * nrow;
   if (rows * 256 > b.msg_elems || rows * 128 > b.dright_elems || rows > b.flag_elems) return false;
   const int ncl = ncol * nrow;
-  hipLaunchKernelGGL(k_cellpar, dim3((ncl + 255) / 256), dim3(256), 0, s, d, cpar, 0);
+  // the constants in T: the FP32 kernels read them as single precision
+  // without a conversion per use (the same values: each is rounded once)
+  T *cp = reinterpret_cast<T *>(cpar);
+  hipLaunchKernelGGL(k_cellpar<T>, dim3((ncl + 255) / 256), dim3(256), 0, s, d, cp, 0);
   b.epoch += 1;
   if (b.ev_before) (void)hipEventRecord(b.ev_before, s);
   // the WENO3 scheme is a separate instantiation so that the NND / UP1
   // kernel's register allocation is untouched by the WENO3 weights
   if (d.scheme == kSchemeWeno3)
-    hipLaunchKernelGGL((k_strip<T, NW, 1>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cpar, F, slot, R,
+    hipLaunchKernelGGL((k_strip<T, NW, 1>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cp, F, slot, R,
                        store_rhs, ntiles, (T *)b.msg, (T *)b.dright, b.flag, b.epoch, b.err, b.oflags);
   else
-    hipLaunchKernelGGL((k_strip<T, NW, 0>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cpar, F, slot, R,
+    hipLaunchKernelGGL((k_strip<T, NW, 0>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cp, F, slot, R,
                        store_rhs, ntiles, (T *)b.msg, (T *)b.dright, b.flag, b.epoch, b.err, b.oflags);
   if (b.ev_after) (void)hipEventRecord(b.ev_after, s);
   if (ngroups > 1) {
     const size_t nthr = rows * 64;
-    hipLaunchKernelGGL(k_seam<T>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, d, cpar, F,
+    hipLaunchKernelGGL(k_seam<T>, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, d, cp, F,
                        (const T *)b.msg, (const T *)b.dright, ntiles, NW);
   }
   return true;
@@ -1863,7 +1868,7 @@ bool launch_walk(const Dev &d, double *cpar, T *F, const T *slot, T *R, int stor
   const size_t slot_bytes = (size_t)8 * d.L * d.nv * sizeof(T);
   if (so < (ptrdiff_t)(2 * (size_t)d.ncell * d.nv * sizeof(T)) || (size_t)so + slot_bytes > 4294967295u) return false;
   const int ncl = (d.nx - 1) * (d.ny - 1);
-  hipLaunchKernelGGL(k_cellpar, dim3((ncl + 255) / 256), dim3(256), 0, s, d, cpar, 3 * vt * (d.nx - 1));
+  hipLaunchKernelGGL(k_cellpar<double>, dim3((ncl + 255) / 256), dim3(256), 0, s, d, cpar, 3 * vt * (d.nx - 1));
   const int ntiles = 2 * ((d.nv + vt - 1) / vt);
   const int nb = (ntiles + K - 1) / K;
   const unsigned planeb = (unsigned)((size_t)d.ncell * d.nv * sizeof(T));
