@@ -1123,7 +1123,14 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
   // dword from each 64-byte line of the 160-byte record), so that the
   // scalar loads of the row's 20 constants hit the scalar cache.
   int pf0 = 0, pf1 = 0, pf2 = 0, pf3 = 0;
-  for (int s = 0; s < nsteps; s++) {
+  // Steps s < w0 and s >= w0 + LT + 2 only keep the workgroup's barriers.
+  // The active steps q = s - w0 = 0 .. LT+1 run five at a time, each given
+  // the five-row windows in rotated order: moving a window down one row
+  // then renames registers instead of copying four values.
+  auto step = [&](int q, T &gA, T &gB, T &gC, T &gD, T &gE, T &hA, T &hB, T &hC, T &hD, T &hE)
+                  __attribute__((always_inline)) {
+    T &g0 = gB, &g1 = gC, &g2 = gD, &g3 = gE, &g4 = gA;  // G(i, r-2..r+2); gA takes the new row
+    T &h0 = hB, &h1 = hC, &h2 = hD, &h3 = hE, &h4 = hA;
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the previous step's copies, loads and stores are complete
     // per-wavefront values, re-derived every step (see fresh())
     const int w = fresh(w0), i = fresh(i0), g = fresh(g0w), cfl = fresh(cfl0);
@@ -1137,197 +1144,202 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
     const int lane = lane_of();
     const unsigned voff = (unsigned)min(tile * 64 + lane, nv - 1) * ES, loff = (unsigned)lane * ES;
     const bool act = tile * 64 + lane < nv;
-    const int q = s - w, r = q - 3;
+    const int s = q + w, r = q - 3;
     const int buf = s & 1, pbuf = buf ^ 1;
     if (producer && flag_due) {  // its message stores are complete: publish the row
       if (lane == 0) st_agent(flag + sbp + (r - 2), epoch);
       flag_due = 0;
     }
-    if (q >= 0 && q < LT + 2) {
-      asm volatile("" ::"s"(pf0), "s"(pf1), "s"(pf2), "s"(pf3));  // the previous step's touches
-      if (r + 1 >= 1 && r + 1 <= nrow) {
-        const int *pc = reinterpret_cast<const int *>(reinterpret_cast<const char *>(cpar) + ocp + ccstep);
-        pf0 = pc[0];
-        pf1 = pc[16];
-        pf2 = pc[kRec > 128 ? 32 : 0];
-        pf3 = pc[kRec / 4 - 1];
-      }
-      const bool real = r >= 1 && r <= nrow;
-      const int rfl = real ? oflags[r] : 0;  // i-face overrides of row r
-      // the values staged in the previous step for this row
-      // (slots that were not copied for this row hold older values, which are
-      // read here but never used)
-      g0 = g1; g1 = g2; g2 = g3; g3 = g4;
-      h0 = h1; h1 = h2; h2 = h3; h3 = h4;
-      g4 = STG[w][0][lane];
-      h4 = STG[w][0][64 + lane];
-      const T cgm1 = STG[w][1][lane], chm1 = STG[w][1][64 + lane];
-      const T cgp1 = STG[w][2][lane], chp1 = STG[w][2][64 + lane];
-      const T cgp2 = STG[w][3][lane], chp2 = STG[w][3][64 + lane];
-      const T cgxs = STG[w][4][lane], chxs = STG[w][4][64 + lane];
-      const T cgys = STG[w][5][lane], chys = STG[w][5][64 + lane];
-      const T cmfg = mfg, cmdg = mdg, cmfh = mfh, cmdh = mdh;
-      // row r: equilibrium, fluxes, RHS, forward recursion (src/GKUA_Residual.f90:39-57,117-469)
-      T gd_new = T(0), hd_new = T(0), bx_new = T(0), by_new = T(0);
-      T feqg = T(0), feqh = T(0), axl = T(0), axr = T(0), ayb = T(0), ayt = T(0), ra = T(0);
-      T vinv = T(0), cf = T(0), vol = T(0);
-      if (real) {
-        const T *cc = reinterpret_cast<const T *>(reinterpret_cast<const char *>(cpar) + ocp);
-        {  // Shakhov equilibrium of cell (i,r) (Get_Feq, src/GKUA_Residual.f90:420-469)
-          const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
-          const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
-          const T fvx = T(cc[1]) * (viu * T(cc[6]) + vjv * T(cc[7]));
-          const T gd = T(cc[0]) * wexp(-vij2);
-          const T b = fvx * (T(2) * vij2 - T(2));
-          // G: gd (1 + phi (2c^2 - 4)); H: (T gd / 2)(1 + phi (2c^2 - 2))
-          feqg = gd * ((T(1) - T(2) * fvx) + b);
-          feqh = ((T(cc[3]) * gd) * T(0.5)) * (T(1) + b);
-        }
-        const T *cc2 = cc;
-        axl = vx * T(cc2[12]) + vy * T(cc2[13]);
-        axr = vx * T(cc2[14]) + vy * T(cc2[15]);
-        ayb = vx * T(cc2[16]) + vy * T(cc2[17]);
-        ayt = vx * T(cc2[18]) + vy * T(cc2[19]);
-        const T alfa = T(cc2[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
-        ra = rcp_pos(alfa);
-        vinv = T(cc2[9]);
-        cf = T(cc2[8]);
-        vol = T(cc2[10]);
-      }
-      // all staged values are in registers before the copies of the next row overwrite them
-      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-      // copies of the next row rn (one step ahead)
-      const int rn = r + 1;
-      long long stF, stS;
-      copy_offsets(lane, stF, stS);
-      auto slotrow = [&](int face, int pos) { return Sb + (((long long)(face - 1) * d.L + pos) * 2) * cb; };
-      if (rn <= nrow) copy(w, 0, Fb + oc + 3 * rb, stF, lane);
-      if (rn >= 1 && rn <= nrow) {
-        copy(w, 1, Fb + oc + rb - cb, stF, lane);
-        copy(w, 2, Fb + oc + rb + cb, stF, lane);
-        copy(w, 3, Fb + oc + rb + 2 * cb, stF, lane);
-        const int rfn = oflags[rn];
-        if (xlo) {
-          if (rfn & 1) copy(w, 4, slotrow(1, rn + 1), stS, lane);
-          else copy(w, 4, Fb + oc + rb - 2 * cb, stF, lane);  // ghost cell (-1, rn)
-        } else if (xhi && (rfn & 2)) {
-          copy(w, 4, slotrow(3, rn + 1), stS, lane);
-        }
-        if (rn == 1 && (cfl & 1)) copy(w, 5, slotrow(2, i + 1), stS, lane);
-        else if (rn == nrow && (cfl & 2)) copy(w, 5, slotrow(4, i + 1), stS, lane);
-        if (consumer) {
-          // the flag of row rn was loaded one step ago; wait for it if the
-          // producing strip is behind, then load the row's message
-          const size_t sr = sbc + (rn - 1);
-          int fv = __builtin_amdgcn_readfirstlane(mflag);
-          if (fv != epoch && __builtin_amdgcn_readfirstlane(ld_agent(werr)) == 0) {
-            int it = 0;
-            do {
-              __builtin_amdgcn_s_sleep(2);
-              fv = __builtin_amdgcn_readfirstlane(ld_agent(flag + sr));
-            } while (fv != epoch && ++it < kSpinMax);
-            if (fv != epoch && lane == 0) st_agent(werr, 1);
-          }
-          const char *m = reinterpret_cast<const char *>(msg + sr * 256) + loff;
-          mfg = ld_agent(reinterpret_cast<const T *>(m));
-          mdg = ld_agent(reinterpret_cast<const T *>(m + 64 * ES));
-          mfh = ld_agent(reinterpret_cast<const T *>(m + 128 * ES));
-          mdh = ld_agent(reinterpret_cast<const T *>(m + 192 * ES));
-        }
-      }
-      if (consumer && rn + 1 >= 1 && rn + 1 <= nrow) mflag = ld_agent(flag + sbc + rn);
-      if (real) {
-        const bool first = r == 1;
-        const bool yrule = first || r >= nrow - 1;  // an overridden j-face within reach of the top face
-        // left face: boundary flux, the seam message or column i-1's value
-        T gfl, gdl, hfl, hdl;
-        if (xlo) {
-          gfl = wflux<SL>(1, nx, rfl, axl, cgxs, cgm1, g2, cgp1, cgxs, hs);
-          hfl = wflux<SL>(1, nx, rfl, axl, chxs, chm1, h2, chp1, chxs, hs);
-          gdl = T(0);
-          hdl = T(0);
-        } else if (w == 0) {
-          gfl = cmfg; gdl = cmdg; hfl = cmfh; hdl = cmdh;
-        } else {
-          const T2 lg = XG[pbuf][w - 1][lane], lh = XH[pbuf][w - 1][lane];
-          gfl = lg.x; gdl = lg.y; hfl = lh.x; hdl = lh.y;
-        }
-        T gfr, hfr, gft, hft, gfb, hfb;
-        if (xrule) {
-          gfr = wflux<SL>(i + 1, nx, rfl, axr, cgm1, g2, cgp1, cgp2, cgxs, hs);
-          hfr = wflux<SL>(i + 1, nx, rfl, axr, chm1, h2, chp1, chp2, chxs, hs);
-        } else {
-          gfr = iflux<SL>(axr, cgm1, g2, cgp1, cgp2, hs);
-          hfr = iflux<SL>(axr, chm1, h2, chp1, chp2, hs);
-        }
-        if (yrule) {
-          gft = wflux<SL>(r + 1, ny, cfl, ayt, g1, g2, g3, g4, cgys, hs);
-          hft = wflux<SL>(r + 1, ny, cfl, ayt, h1, h2, h3, h4, chys, hs);
-        } else {
-          gft = iflux<SL>(ayt, g1, g2, g3, g4, hs);
-          hft = iflux<SL>(ayt, h1, h2, h3, h4, hs);
-        }
-        if (first) {
-          gfb = wflux<SL>(1, ny, cfl, ayb, g0, g1, g2, g3, cgys, hs);
-          hfb = wflux<SL>(1, ny, cfl, ayb, h0, h1, h2, h3, chys, hs);
-        } else {
-          gfb = gfly;
-          hfb = hfly;
-        }
-        gfly = gft;
-        hfly = hft;
-        const T grr = -(((gfr - gfl) + gft) - gfb) * vinv + cf * (feqg - g2);
-        const T hrr = -(((hfr - hfl) + hft) - hfb) * vinv + cf * (feqh - h2);
-        if (store_rhs && act) {
-          stv(reinterpret_cast<T *>(Rb + oc), voff, grr);
-          stv(reinterpret_cast<T *>(Rb + oc + pb), voff, hrr);
-        }
-        gd_new = (vol * grr + posp(axl) * gdl + posp(ayb) * (first ? T(0) : gd_n)) * ra;
-        hd_new = (vol * hrr + posp(axl) * hdl + posp(ayb) * (first ? T(0) : hd_n)) * ra;
-        XG[buf][w][lane] = T2{gfr, gd_new};
-        XH[buf][w][lane] = T2{hfr, hd_new};
-        bx_new = -negp(axr) * ra;
-        by_new = -negp(ayt) * ra;
-        if (producer) {
-          char *m = reinterpret_cast<char *>(msg + (sbp + (r - 1)) * 256) + loff;
-          st_agent(reinterpret_cast<T *>(m), gfr);
-          st_agent(reinterpret_cast<T *>(m + 64 * ES), gd_new);
-          st_agent(reinterpret_cast<T *>(m + 128 * ES), hfr);
-          st_agent(reinterpret_cast<T *>(m + 192 * ES), hd_new);
-          flag_due = 1;
-        }
-        if (consumer) {
-          T *m = dright + (sbc + (r - 1)) * 128;
-          stv(m, loff, gd_new);
-          stv(m + 64, loff, hd_new);
-        }
-      }
-      // backward pass + update of row r-2 (src/GKUA_Residual.f90:64-109)
-      const int rf = r - 2;
-      if (rf >= 1 && rf <= nrow && !producer) {
-        T gr = T(0), hr = T(0);
-        if (i < ncol) {
-          gr = XG[pbuf][w + 1][lane].y;
-          hr = XH[pbuf][w + 1][lane].y;
-        }
-        const bool last = rf == nrow;  // Delta*(i, ny) = 0
-        const T gu = gd_o + (bx_o * gr + by_o * (last ? T(0) : gd_n));
-        const T hu = hd_o + (bx_o * hr + by_o * (last ? T(0) : hd_n));
-        T fg = g0 + gu, fh = h0 + hu;
-        if ((double)fg < kClamp) fg = T(kClamp);
-        if ((double)fh < kClamp) fh = T(kClamp);
-        if (act) {
-          stv(reinterpret_cast<T *>(Fb + oc - 2 * rb), voff, fg);
-          stv(reinterpret_cast<T *>(Fb + oc - 2 * rb + pb), voff, fh);
-        }
-      }
-      gd_o = gd_n; hd_o = hd_n; bx_o = bx_n; by_o = by_n;
-      gd_n = gd_new; hd_n = hd_new; bx_n = bx_new; by_n = by_new;
-      oc += rb;
-      ocp += ccstep;
+    asm volatile("" ::"s"(pf0), "s"(pf1), "s"(pf2), "s"(pf3));  // the previous step's touches
+    if (r + 1 >= 1 && r + 1 <= nrow) {
+      const int *pc = reinterpret_cast<const int *>(reinterpret_cast<const char *>(cpar) + ocp + ccstep);
+      pf0 = pc[0];
+      pf1 = pc[16];
+      pf2 = pc[kRec > 128 ? 32 : 0];
+      pf3 = pc[kRec / 4 - 1];
     }
+    const bool real = r >= 1 && r <= nrow;
+    const int rfl = real ? oflags[r] : 0;  // i-face overrides of row r
+    // the values staged in the previous step for this row
+    // (slots that were not copied for this row hold older values, which are
+    // read here but never used)
+    g4 = STG[w][0][lane];
+    h4 = STG[w][0][64 + lane];
+    const T cgm1 = STG[w][1][lane], chm1 = STG[w][1][64 + lane];
+    const T cgp1 = STG[w][2][lane], chp1 = STG[w][2][64 + lane];
+    const T cgp2 = STG[w][3][lane], chp2 = STG[w][3][64 + lane];
+    const T cgxs = STG[w][4][lane], chxs = STG[w][4][64 + lane];
+    const T cgys = STG[w][5][lane], chys = STG[w][5][64 + lane];
+    const T cmfg = mfg, cmdg = mdg, cmfh = mfh, cmdh = mdh;
+    // row r: equilibrium, fluxes, RHS, forward recursion (src/GKUA_Residual.f90:39-57,117-469)
+    T gd_new = T(0), hd_new = T(0), bx_new = T(0), by_new = T(0);
+    T feqg = T(0), feqh = T(0), axl = T(0), axr = T(0), ayb = T(0), ayt = T(0), ra = T(0);
+    T vinv = T(0), cf = T(0), vol = T(0);
+    if (real) {
+      const T *cc = reinterpret_cast<const T *>(reinterpret_cast<const char *>(cpar) + ocp);
+      {  // Shakhov equilibrium of cell (i,r) (Get_Feq, src/GKUA_Residual.f90:420-469)
+        const T viu = vx - T(cc[4]), vjv = vy - T(cc[5]);
+        const T vij2 = (viu * viu + vjv * vjv) * T(cc[2]);
+        const T fvx = T(cc[1]) * (viu * T(cc[6]) + vjv * T(cc[7]));
+        const T gd = T(cc[0]) * wexp(-vij2);
+        const T b = fvx * (T(2) * vij2 - T(2));
+        // G: gd (1 + phi (2c^2 - 4)); H: (T gd / 2)(1 + phi (2c^2 - 2))
+        feqg = gd * ((T(1) - T(2) * fvx) + b);
+        feqh = ((T(cc[3]) * gd) * T(0.5)) * (T(1) + b);
+      }
+      const T *cc2 = cc;
+      axl = vx * T(cc2[12]) + vy * T(cc2[13]);
+      axr = vx * T(cc2[14]) + vy * T(cc2[15]);
+      ayb = vx * T(cc2[16]) + vy * T(cc2[17]);
+      ayt = vx * T(cc2[18]) + vy * T(cc2[19]);
+      const T alfa = T(cc2[11]) + (posp(axr) - negp(axl) + posp(ayt) - negp(ayb));
+      ra = rcp_pos(alfa);
+      vinv = T(cc2[9]);
+      cf = T(cc2[8]);
+      vol = T(cc2[10]);
+    }
+    // all staged values are in registers before the copies of the next row overwrite them
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    // copies of the next row rn (one step ahead)
+    const int rn = r + 1;
+    long long stF, stS;
+    copy_offsets(lane, stF, stS);
+    auto slotrow = [&](int face, int pos) { return Sb + (((long long)(face - 1) * d.L + pos) * 2) * cb; };
+    if (rn <= nrow) copy(w, 0, Fb + oc + 3 * rb, stF, lane);
+    if (rn >= 1 && rn <= nrow) {
+      copy(w, 1, Fb + oc + rb - cb, stF, lane);
+      copy(w, 2, Fb + oc + rb + cb, stF, lane);
+      copy(w, 3, Fb + oc + rb + 2 * cb, stF, lane);
+      const int rfn = oflags[rn];
+      if (xlo) {
+        if (rfn & 1) copy(w, 4, slotrow(1, rn + 1), stS, lane);
+        else copy(w, 4, Fb + oc + rb - 2 * cb, stF, lane);  // ghost cell (-1, rn)
+      } else if (xhi && (rfn & 2)) {
+        copy(w, 4, slotrow(3, rn + 1), stS, lane);
+      }
+      if (rn == 1 && (cfl & 1)) copy(w, 5, slotrow(2, i + 1), stS, lane);
+      else if (rn == nrow && (cfl & 2)) copy(w, 5, slotrow(4, i + 1), stS, lane);
+      if (consumer) {
+        // the flag of row rn was loaded one step ago; wait for it if the
+        // producing strip is behind, then load the row's message
+        const size_t sr = sbc + (rn - 1);
+        int fv = __builtin_amdgcn_readfirstlane(mflag);
+        if (fv != epoch && __builtin_amdgcn_readfirstlane(ld_agent(werr)) == 0) {
+          int it = 0;
+          do {
+            __builtin_amdgcn_s_sleep(2);
+            fv = __builtin_amdgcn_readfirstlane(ld_agent(flag + sr));
+          } while (fv != epoch && ++it < kSpinMax);
+          if (fv != epoch && lane == 0) st_agent(werr, 1);
+        }
+        const char *m = reinterpret_cast<const char *>(msg + sr * 256) + loff;
+        mfg = ld_agent(reinterpret_cast<const T *>(m));
+        mdg = ld_agent(reinterpret_cast<const T *>(m + 64 * ES));
+        mfh = ld_agent(reinterpret_cast<const T *>(m + 128 * ES));
+        mdh = ld_agent(reinterpret_cast<const T *>(m + 192 * ES));
+      }
+    }
+    if (consumer && rn + 1 >= 1 && rn + 1 <= nrow) mflag = ld_agent(flag + sbc + rn);
+    if (real) {
+      const bool first = r == 1;
+      const bool yrule = first || r >= nrow - 1;  // an overridden j-face within reach of the top face
+      // left face: boundary flux, the seam message or column i-1's value
+      T gfl, gdl, hfl, hdl;
+      if (xlo) {
+        gfl = wflux<SL>(1, nx, rfl, axl, cgxs, cgm1, g2, cgp1, cgxs, hs);
+        hfl = wflux<SL>(1, nx, rfl, axl, chxs, chm1, h2, chp1, chxs, hs);
+        gdl = T(0);
+        hdl = T(0);
+      } else if (w == 0) {
+        gfl = cmfg; gdl = cmdg; hfl = cmfh; hdl = cmdh;
+      } else {
+        const T2 lg = XG[pbuf][w - 1][lane], lh = XH[pbuf][w - 1][lane];
+        gfl = lg.x; gdl = lg.y; hfl = lh.x; hdl = lh.y;
+      }
+      T gfr, hfr, gft, hft, gfb, hfb;
+      if (xrule) {
+        gfr = wflux<SL>(i + 1, nx, rfl, axr, cgm1, g2, cgp1, cgp2, cgxs, hs);
+        hfr = wflux<SL>(i + 1, nx, rfl, axr, chm1, h2, chp1, chp2, chxs, hs);
+      } else {
+        gfr = iflux<SL>(axr, cgm1, g2, cgp1, cgp2, hs);
+        hfr = iflux<SL>(axr, chm1, h2, chp1, chp2, hs);
+      }
+      if (yrule) {
+        gft = wflux<SL>(r + 1, ny, cfl, ayt, g1, g2, g3, g4, cgys, hs);
+        hft = wflux<SL>(r + 1, ny, cfl, ayt, h1, h2, h3, h4, chys, hs);
+      } else {
+        gft = iflux<SL>(ayt, g1, g2, g3, g4, hs);
+        hft = iflux<SL>(ayt, h1, h2, h3, h4, hs);
+      }
+      if (first) {
+        gfb = wflux<SL>(1, ny, cfl, ayb, g0, g1, g2, g3, cgys, hs);
+        hfb = wflux<SL>(1, ny, cfl, ayb, h0, h1, h2, h3, chys, hs);
+      } else {
+        gfb = gfly;
+        hfb = hfly;
+      }
+      gfly = gft;
+      hfly = hft;
+      const T grr = -(((gfr - gfl) + gft) - gfb) * vinv + cf * (feqg - g2);
+      const T hrr = -(((hfr - hfl) + hft) - hfb) * vinv + cf * (feqh - h2);
+      if (store_rhs && act) {
+        stv(reinterpret_cast<T *>(Rb + oc), voff, grr);
+        stv(reinterpret_cast<T *>(Rb + oc + pb), voff, hrr);
+      }
+      gd_new = (vol * grr + posp(axl) * gdl + posp(ayb) * (first ? T(0) : gd_n)) * ra;
+      hd_new = (vol * hrr + posp(axl) * hdl + posp(ayb) * (first ? T(0) : hd_n)) * ra;
+      XG[buf][w][lane] = T2{gfr, gd_new};
+      XH[buf][w][lane] = T2{hfr, hd_new};
+      bx_new = -negp(axr) * ra;
+      by_new = -negp(ayt) * ra;
+      if (producer) {
+        char *m = reinterpret_cast<char *>(msg + (sbp + (r - 1)) * 256) + loff;
+        st_agent(reinterpret_cast<T *>(m), gfr);
+        st_agent(reinterpret_cast<T *>(m + 64 * ES), gd_new);
+        st_agent(reinterpret_cast<T *>(m + 128 * ES), hfr);
+        st_agent(reinterpret_cast<T *>(m + 192 * ES), hd_new);
+        flag_due = 1;
+      }
+      if (consumer) {
+        T *m = dright + (sbc + (r - 1)) * 128;
+        stv(m, loff, gd_new);
+        stv(m + 64, loff, hd_new);
+      }
+    }
+    // backward pass + update of row r-2 (src/GKUA_Residual.f90:64-109)
+    const int rf = r - 2;
+    if (rf >= 1 && rf <= nrow && !producer) {
+      T gr = T(0), hr = T(0);
+      if (i < ncol) {
+        gr = XG[pbuf][w + 1][lane].y;
+        hr = XH[pbuf][w + 1][lane].y;
+      }
+      const bool last = rf == nrow;  // Delta*(i, ny) = 0
+      const T gu = gd_o + (bx_o * gr + by_o * (last ? T(0) : gd_n));
+      const T hu = hd_o + (bx_o * hr + by_o * (last ? T(0) : hd_n));
+      T fg = g0 + gu, fh = h0 + hu;
+      if ((double)fg < kClamp) fg = T(kClamp);
+      if ((double)fh < kClamp) fh = T(kClamp);
+      if (act) {
+        stv(reinterpret_cast<T *>(Fb + oc - 2 * rb), voff, fg);
+        stv(reinterpret_cast<T *>(Fb + oc - 2 * rb + pb), voff, fh);
+      }
+    }
+    gd_o = gd_n; hd_o = hd_n; bx_o = bx_n; by_o = by_n;
+    gd_n = gd_new; hd_n = hd_new; bx_n = bx_new; by_n = by_new;
+    oc += rb;
+    ocp += ccstep;
     barrier_lds();
+  };
+  for (int s = 0; s < w0; s++) barrier_lds();
+  for (int q = 0; q < LT + 2; q += 5) {
+    step(q, g0, g1, g2, g3, g4, h0, h1, h2, h3, h4);
+    if (q + 1 < LT + 2) step(q + 1, g1, g2, g3, g4, g0, h1, h2, h3, h4, h0);
+    if (q + 2 < LT + 2) step(q + 2, g2, g3, g4, g0, g1, h2, h3, h4, h0, h1);
+    if (q + 3 < LT + 2) step(q + 3, g3, g4, g0, g1, g2, h3, h4, h0, h1, h2);
+    if (q + 4 < LT + 2) step(q + 4, g4, g0, g1, g2, g3, h4, h0, h1, h2, h3);
   }
+  for (int s = w0 + LT + 2; s < nsteps; s++) barrier_lds();
   if (w0 == NW - 1 && g0w < ngroups - 1 && flag_due) {
     __builtin_amdgcn_s_waitcnt(0x0F70);
     if ((threadIdx.x & 63) == 0) st_agent(flag + sbp + (nrow - 1), epoch);
