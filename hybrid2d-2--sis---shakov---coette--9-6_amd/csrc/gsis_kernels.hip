@@ -53,6 +53,39 @@ __device__ __forceinline__ float rcp_pos(float a) {
 //   slope = (bc dn + 2 bn dc) / (bc + 2 bn),  b = (D^2 + eps)^2.
 // eps is the REAL(4) literal 1.E-15 (src/GKUA_Residual.f90:143).
 constexpr int kSchemeWeno3 = 3;  // Iflag_scheme of WENO3 (src/COMMONDATA.f90:18)
+// Sum of x over the 64 lanes of a wavefront, valid in lane 63, by DPP moves
+// (quad_perm, half-row and row mirrors, then row_bcast:15 and row_bcast:31
+// into the upper rows) instead of lane shuffles through LDS.
+__device__ __forceinline__ double dpp_mov(double x, int ctrl, int row_mask) {
+  const unsigned long long b = __double_as_longlong(x);
+  int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+  switch (ctrl) {  // the DPP control must be a constant
+    case 0xB1: lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
+               hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false); break;
+    case 0x4E: lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);
+               hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false); break;
+    case 0x141: lo = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xF, 0xF, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xF, 0xF, false); break;
+    case 0x140: lo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xF, 0xF, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xF, 0xF, false); break;
+    case 0x142: lo = __builtin_amdgcn_update_dpp(0, lo, 0x142, 0xA, 0xF, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x142, 0xA, 0xF, false); break;
+    default: lo = __builtin_amdgcn_update_dpp(0, lo, 0x143, 0xC, 0xF, false);
+             hi = __builtin_amdgcn_update_dpp(0, hi, 0x143, 0xC, 0xF, false); break;
+  }
+  (void)row_mask;
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double wave_sum_last(double x) {
+  x += dpp_mov(x, 0xB1, 0xF);   // quad_perm [1,0,3,2]
+  x += dpp_mov(x, 0x4E, 0xF);   // quad_perm [2,3,0,1]: quad sums
+  x += dpp_mov(x, 0x141, 0xF);  // row_half_mirror: sums of 8
+  x += dpp_mov(x, 0x140, 0xF);  // row_mirror: row sums of 16
+  x += dpp_mov(x, 0x142, 0xA);  // row_bcast:15 into rows 1, 3
+  x += dpp_mov(x, 0x143, 0xC);  // row_bcast:31 into rows 2, 3
+  return x;
+}
+
 constexpr double kEpsWeno = 1.0000000036274937e-15;  // (double)1.0e-15f
 template <typename T>
 __device__ __forceinline__ T wslope(T dc, T dn) {
@@ -1436,48 +1469,69 @@ __global__ void __launch_bounds__(256) k_moments(Dev d, T *FG, T *FH) {
   const int nvi = d.nvi, nvj = d.nvj;
   const int R = nvi <= 64 ? 64 / nvi : 1;  // rows per pass
   const int nch = nvi <= 64 ? 1 : (nvi + 63) / 64;
+  // A lane keeps one velocity column iv, so vx and wx are fixed along its
+  // rows and the ten moments factor into six sums over the rows:
+  //   A0 = sum wy G, A1 = sum wy vy G, A2 = sum wy vy^2 G, A5 = sum wy vy^3 G,
+  //   A3 = sum wy H, A4 = sum wy vy H,
+  // one FMA each per velocity (the row factors wy vy^k are per row).
   double acc[10];
 #pragma unroll
   for (int k = 0; k < 10; k++) acc[k] = 0.0;
+  // R = 1 (nvi > 32): one row per pass, so the row index and its factors are
+  // wavefront-uniform (scalar loads)
+  const bool uni = R == 1;
   for (int ch = 0; ch < nch; ch++) {
     const int iv = nvi <= 64 ? lane % nvi : ch * 64 + lane;
-    const int jr = nvi <= 64 ? lane / nvi : 0;
-    if (iv >= nvi || jr >= R) continue;
+    const int jr = uni ? 0 : lane / nvi;
+    if (iv >= nvi || (uni ? (nvi <= 64 && lane >= nvi) : jr >= R)) continue;
     const double vx = d.vi[iv], wx = d.wi[iv];
-    // unrolled so that several rows' loads are in flight per wavefront (the
-    // sums still run over jv in order)
-#pragma unroll 4
-    for (int jv = jr; jv < nvj; jv += R) {
+    double A0 = 0.0, A1 = 0.0, A2 = 0.0, A3 = 0.0, A4 = 0.0, A5 = 0.0;
+    auto add = [&](int jv) {
       const size_t ov = o + (size_t)jv * nvi + iv;
       T g = FG[ov], h = FH[ov];
       if (!interior) {
         if ((double)g < kClamp) { g = T(kClamp); FG[ov] = g; }
         if ((double)h < kClamp) { h = T(kClamp); FH[ov] = h; }
       }
-      const double vy = d.vj[jv], w = wx * d.wj[jv];
+      const double vy = d.vj[jv], wy = d.wj[jv];
+      const double w1 = wy * vy, w2 = w1 * vy, w3 = w2 * vy;
       const double gd = (double)g, hd = (double)h;
-      const double vx1 = vx * gd, vy1 = vy * gd, vx2 = vx * vx1, vy2 = vy * vy1;
-      const double f4 = hd + vx2 + vy2;
-      acc[0] += w * gd;
-      acc[1] += w * vx1;
-      acc[2] += w * vy1;
-      acc[3] += w * f4;
-      acc[4] += w * vx2;
-      acc[5] += w * vy2;
-      acc[6] += w * (vx * vy * gd);
-      acc[7] += w * (vx * f4);
-      acc[8] += w * (vy * f4);
-      acc[9] += w * hd;
+      A0 += wy * gd;
+      A1 += w1 * gd;
+      A2 += w2 * gd;
+      A5 += w3 * gd;
+      A3 += wy * hd;
+      A4 += w1 * hd;
+    };
+    if (uni) {
+#pragma unroll 4
+      for (int jv = 0; jv < nvj; jv++) add(jv);
+    } else {
+#pragma unroll 4
+      for (int jv = jr; jv < nvj; jv += R) add(jv);
     }
+    // sum_v w {G, vx G, vy G, H + |v|^2 G, vx^2 G, vy^2 G, vx vy G,
+    //          vx (H + |v|^2 G), vy (H + |v|^2 G), H} of this column
+    const double vx2 = vx * vx;
+    const double e4 = A3 + vx2 * A0 + A2;  // sum wy (H + |v|^2 G)
+    acc[0] += wx * A0;
+    acc[1] += wx * (vx * A0);
+    acc[2] += wx * A1;
+    acc[3] += wx * e4;
+    acc[4] += wx * (vx2 * A0);
+    acc[5] += wx * A2;
+    acc[6] += wx * (vx * A1);
+    acc[7] += wx * (vx * e4);
+    acc[8] += wx * (A4 + vx2 * A1 + A5);
+    acc[9] += wx * A3;
   }
   if (!ring) {
     if (lane < 11) d.S[lane * d.ncell + c] = 0.0;
     return;
   }
 #pragma unroll
-  for (int k = 0; k < 10; k++)
-    for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off, 64);
-  if (lane == 0) {
+  for (int k = 0; k < 10; k++) acc[k] = wave_sum_last(acc[k]);
+  if (lane == 63) {
     double ron = acc[0];
     if (ron <= 0.) ron = 1.0;  // R00
     const double u = acc[1] / ron, v = acc[2] / ron;
