@@ -176,47 +176,69 @@ __device__ void wall_node(const Dev &d, const BcDev &b, int kk, T *FG, T *FH, T 
   st = (b.face <= 2) ? 1.0 : -1.0;
   const size_t c0 = (size_t)cidx(d, px, py) * d.nv, c1 = (size_t)cidx(d, px1, py1) * d.nv;
   const double tw = d.twall;
+  // exp(-|c|^2/Tw) = exp(-cx^2/Tw) exp(-cy^2/Tw): nvi + nvj exponentials per
+  // face node, in LDS
+  __shared__ double ex[kMaxV1], ey[kMaxV1];
+  for (int k = threadIdx.x; k < d.nvi; k += blockDim.x) {
+    const double cx = d.vi[k] - b.uw;
+    ex[k] = exp(-(cx * cx) / tw);
+  }
+  for (int k = threadIdx.x; k < d.nvj; k += blockDim.x) {
+    const double cy = d.vj[k] - b.vw;
+    ey[k] = exp(-(cy * cy) / tw);
+  }
+  __syncthreads();
+  // velocity v = jv*nvi + iv, stepped by blockDim.x without a division
+  const int dj = (int)blockDim.x / d.nvi, di = (int)blockDim.x - dj * d.nvi;
   double acc[2] = {0.0, 0.0};
   int first_in = 0x7fffffff;
-  for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
-    const int iv = v % d.nvi, jv = v / d.nvi;
-    T outg, outh;
-    {
-      T f1 = FG[c0 + v], f2 = FG[c1 + v];
-      T t = (-f2 + T(3) * f1) * T(0.5);
-      if (t <= T(0)) t = f1;
-      if (t >= T(1.5) * f1) t = T(1.5) * f1;
-      outg = t;
-      f1 = FH[c0 + v]; f2 = FH[c1 + v];
-      t = (-f2 + T(3) * f1) * T(0.5);
-      if (t <= T(0)) t = f1;
-      if (t >= T(1.5) * f1) t = T(1.5) * f1;
-      outh = t;
+  {
+    int iv = (int)threadIdx.x % d.nvi, jv = (int)threadIdx.x / d.nvi;
+    for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
+      T outg, outh;
+      {
+        T f1 = FG[c0 + v], f2 = FG[c1 + v];
+        T t = (-f2 + T(3) * f1) * T(0.5);
+        if (t <= T(0)) t = f1;
+        if (t >= T(1.5) * f1) t = T(1.5) * f1;
+        outg = t;
+        f1 = FH[c0 + v]; f2 = FH[c1 + v];
+        t = (-f2 + T(3) * f1) * T(0.5);
+        if (t <= T(0)) t = f1;
+        if (t >= T(1.5) * f1) t = T(1.5) * f1;
+        outh = t;
+      }
+      slot[slot_idx(d, b.face, pos, 0, v)] = outg;
+      slot[slot_idx(d, b.face, pos, 1, v)] = outh;
+      const double w = d.wi[iv] * d.wj[jv];
+      const double cx = d.vi[iv] - b.uw, cy = d.vj[jv] - b.vw;
+      const double vnw = (cx * swnx + cy * swny) * st;
+      const double vnn = (vnw - fabs(vnw)) / 2.;
+      acc[0] += w * (vnn * (double)outg);
+      acc[1] += w * ((vnw + fabs(vnw)) / 2. * (ex[iv] * ey[jv]) / (kPi * tw));
+      if (vnw >= 0.0) first_in = min(first_in, v);
+      iv += di; jv += dj;
+      if (iv >= d.nvi) { iv -= d.nvi; jv += 1; }
     }
-    slot[slot_idx(d, b.face, pos, 0, v)] = outg;
-    slot[slot_idx(d, b.face, pos, 1, v)] = outh;
-    const double w = d.wi[iv] * d.wj[jv];
-    const double cx = d.vi[iv] - b.uw, cy = d.vj[jv] - b.vw;
-    const double vnw = (cx * swnx + cy * swny) * st;
-    const double vnn = (vnw - fabs(vnw)) / 2.;
-    acc[0] += w * (vnn * (double)outg);
-    acc[1] += w * ((vnw + fabs(vnw)) / 2. * exp(-(cx * cx + cy * cy) / tw) / (kPi * tw));
-    if (vnw >= 0.0) first_in = min(first_in, v);
   }
   block_sum<2>(acc);
   first_in = block_min_int(first_in);
   const double rho = -acc[0] / acc[1];
-  for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
-    const int iv = v % d.nvi, jv = v / d.nvi;
-    const double cx = d.vi[iv] - b.uw, cy = d.vj[jv] - b.vw;
-    const double vnw = (cx * swnx + cy * swny) * st;
-    if (vnw >= 0.0) {
-      // the reference clamps a negative wall density only after its first use
-      // (src/GKUA_wall_functions.f90:52-54)
-      const double r = (v == first_in || rho >= 0.0) ? rho : 0.0;
-      const double g = r * exp(-(cx * cx + cy * cy) / tw) / (kPi * tw);
-      slot[slot_idx(d, b.face, pos, 0, v)] = T(g);
-      slot[slot_idx(d, b.face, pos, 1, v)] = T(tw * g / 2.);
+  {
+    int iv = (int)threadIdx.x % d.nvi, jv = (int)threadIdx.x / d.nvi;
+    for (int v = threadIdx.x; v < d.nv; v += blockDim.x) {
+      const double cx = d.vi[iv] - b.uw, cy = d.vj[jv] - b.vw;
+      const double vnw = (cx * swnx + cy * swny) * st;
+      if (vnw >= 0.0) {
+        // the reference clamps a negative wall density only after its first use
+        // (src/GKUA_wall_functions.f90:52-54)
+        const double r = (v == first_in || rho >= 0.0) ? rho : 0.0;
+        const double g = r * (ex[iv] * ey[jv]) / (kPi * tw);
+        slot[slot_idx(d, b.face, pos, 0, v)] = T(g);
+        slot[slot_idx(d, b.face, pos, 1, v)] = T(tw * g / 2.);
+      }
+      iv += di; jv += dj;
+      if (iv >= d.nvi) { iv -= d.nvi; jv += 1; }
     }
   }
 }
