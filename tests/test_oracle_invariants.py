@@ -240,6 +240,31 @@ def test_weno3_wall_adjacent_faces_on_zigzag():
     assert worst < 1e-9, worst
 
 
+@pytest.mark.parametrize("axis", [0, 1], ids=["i", "j"])
+def test_periodic_copy_is_translation_equivariant(axis):
+    """On the doubly periodic box the explicit RHS commutes with a cyclic
+    shift of the cells: the RHS of the data shifted by one cell is the shifted
+    RHS.  The stencil reaches two cells past each face, so every ghost value
+    the periodic copy (GKUA_boundary_inlet, GKUA_boundary.f90:219-273) writes
+    is used, and a copy from the wrong cell breaks the identity near the
+    boundary.  (The LU-SGS sweep starts at i = j = 1 and is not
+    shift-invariant, so the stored RHS is compared, not the update.)"""
+    case = _case_periodic_box(scheme=1)
+    n = case.nx - 1
+    rng = np.random.default_rng(7)
+    inner = 1.0 + 0.02 * rng.standard_normal((n, n))
+    shape = np.ones((case.nx + 7, case.ny + 7))
+    shape[4:4 + n, 4:4 + n] = inner
+    R, g0 = _rhs_of(case, shape)
+    shape2 = np.ones_like(shape)
+    shape2[4:4 + n, 4:4 + n] = np.roll(inner, 1, axis=axis)
+    R2, _ = _rhs_of(case, shape2)
+    a = np.roll(R[:, 4:4 + n, 4:4 + n], 1, axis=1 + axis)
+    b = R2[:, 4:4 + n, 4:4 + n]
+    err = np.max(np.abs(a - b)) / np.max(np.abs(a))
+    assert err < 1e-10, err
+
+
 def test_wall_zero_mass_flux():
     g = load_pkg()
     case = g.cases.cavity_case(8, 16, kn=0.1, lid=0.2, dt=0.5)
