@@ -154,6 +154,38 @@ def test_weno3_ideal_weights_on_zigzag(scheme, gain):
     assert worst < 1e-9, worst
 
 
+def test_residual_closed_form_on_zigzag():
+    """DETFP_res (GKUA_DETFP_RES.f90:4-60, 81-120) on the zigzag above: with
+    RHS = -n gain g0 (zx s_i |vx| + zy s_j |vy|) in every cell of the periodic
+    box (n even, so the zigzag is periodic), the four velocity moments of the
+    RHS are -n gain (zx s_i A + zy s_j B) with A = sum w phi |vx| X,
+    B = sum w phi |vy| X, where phi = 1, vx, vy, 1 and X = G0, G0, G0,
+    H0 + |v|^2 G0 (the momenta do not vanish: the composite Newton-Cotes
+    weights of an odd interval count are not symmetric); averaged over the
+    cells the cross term s_i s_j vanishes, so Res_rms = n gain
+    sqrt((zx A)^2 + (zy B)^2)."""
+    zx, zy, gain = 1e-2, 3e-3, 4.0 / 3.0
+    case = _case_periodic_box(scheme=3)
+    ii = np.arange(-3, case.nx + 4)[:, None]
+    jj = np.arange(-3, case.ny + 4)[None, :]
+    o = load_oracle().OracleSolver(case)
+    o.initialize()
+    g0 = o.frd()
+    o.upload(frd=g0 * (1.0 + zx * (-1.0) ** ii + zy * (-1.0) ** jj)[None, :, :, None, None])
+    o.step(1)
+    res = o.residual()
+    o.close()
+    n = case.nx - 1
+    W = np.outer(case.wi, case.wj)
+    VX, VY = np.meshgrid(case.vi, case.vj, indexing="ij")
+    G, H = g0[0, 5, 5], g0[1, 5, 5]
+    want = []
+    for phi, X in ((1.0, G), (VX, G), (VY, G), (1.0, H + (VX ** 2 + VY ** 2) * G)):
+        a, b = (np.sum(W * phi * np.abs(V) * X) for V in (VX, VY))
+        want.append(n * gain * math.hypot(zx * a, zy * b))
+    assert np.allclose(res, want, rtol=1e-9, atol=0.0), (res, want)
+
+
 @pytest.mark.parametrize("axis", [0, 1], ids=["i", "j"])
 @pytest.mark.parametrize("where", ["box", "cavity_low", "cavity_high"])
 def test_weno3_step_takes_smooth_stencil(where, axis):
