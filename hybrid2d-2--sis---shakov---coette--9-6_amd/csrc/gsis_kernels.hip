@@ -614,7 +614,7 @@ __global__ void k_cellpar(Dev d, P *cpar, int pstride) {
 // flux is a * slot (src/GKUA_Residual.f90:200-262).
 // hs is 1/2 for the NND and WENO3 schemes and 0 for first-order upwind (the
 // limited slopes then drop out exactly); SL picks the slope rule (0 minmod,
-// 1 WENO3).
+// 1 WENO3, 2 none).
 // The two limited slopes of a face, s(d0, dl) and s(d0, dr).  For minmod in
 // FP64, minmod(d0, x) = clamp(x, min(d0, 0), max(d0, 0)) (the same value: x
 // when it lies between 0 and d0, else the nearer bound), so the bounds are
@@ -622,7 +622,10 @@ __global__ void k_cellpar(Dev d, P *cpar, int pstride) {
 // v_med3_f32, one instruction per slope.
 template <int SL, typename T>
 __device__ __forceinline__ void slopes(T d0, T dl, T dr, T &sl, T &sr) {
-  if constexpr (SL == 0 && std::is_same<T, double>::value) {
+  if constexpr (SL == 2) {  // first-order upwind: no slopes
+    sl = T(0);
+    sr = T(0);
+  } else if constexpr (SL == 0 && std::is_same<T, double>::value) {
     const T lo = fmin(d0, T(0)), hi = fmax(d0, T(0));
     sl = fmin(fmax(dl, lo), hi);
     sr = fmin(fmax(dr, lo), hi);
@@ -646,6 +649,7 @@ __device__ __forceinline__ T wflux(int f, int n, int fl, T a, T fm2, T fm1, T f0
 // interior face (no override) of the fused kernel's fast path
 template <int SL, typename T>
 __device__ __forceinline__ T iflux(T a, T fm2, T fm1, T f0, T fp1, T hs) {
+  if constexpr (SL == 2) return posp(a) * fm1 + negp(a) * f0;
   const T d0 = f0 - fm1;
   T sl, sr;
   slopes<SL>(d0, fm1 - fm2, fp1 - f0, sl, sr);
@@ -1079,7 +1083,7 @@ __device__ __forceinline__ void lds_copy16(const void *src, void *dst) {
 #endif
 }
 
-// SL: slope rule of the fluxes (0 minmod for NND / UP1, 1 WENO3).  At least
+// SL: slope rule of the fluxes (0 minmod for NND, 1 WENO3, 2 none: UP1).  At least
 // 4 wavefronts per SIMD (at most 128 VGPRs): with 80 KB of LDS per 8-column
 // workgroup two workgroups share a CU, which the register file must allow.
 template <typename T, int NW, int SL>
@@ -1120,7 +1124,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
     vx = T(d.vi[vv % d.nvi]);
     vy = T(d.vj[vv / d.nvi]);
   }
-  const T hs = d.scheme != 0 ? T(0.5) : T(0);
+  constexpr T hs = SL == 2 ? T(0) : T(0.5);  // slope factor (0: first-order upwind)
   const int cfl0 = oflags[ny + 2 + i0];  // j-face overrides of column i
   // Roles, re-derived every step: the first column of strip g > 0 reads the
   // seam message of strip g-1 (consumer); the last column of a strip that is
@@ -1159,10 +1163,11 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
   int mflag = 0;                                     // flag of row r+2
   long long oc = -2 * rb + (long long)(i0 + 1) * cb;  // byte offset of cell (i, r), r = -3
   constexpr long long kRec = kNW * (long long)sizeof(T);        // bytes of one cell's constants
-  long long ocp = ((long long)(-4) * ncol + (i0 - 1)) * kRec;  // cpar byte offset of cell (i, r)
-  const long long ccstep = (long long)ncol * kRec;
-  const size_t sbc = ((size_t)(g0w > 0 ? g0w - 1 : 0) * ntiles + tile) * (size_t)nrow;  // seam rows read (row r at sbc + r - 1)
-  const size_t sbp = ((size_t)g0w * ntiles + tile) * (size_t)nrow;                    // seam rows written
+  // 32-bit offsets and indices where they fit, to keep scalar registers free
+  int ocp = (-4 * ncol + (i0 - 1)) * (int)kRec;  // cpar byte offset of cell (i, r); used for r >= 1
+  const int ccstep = ncol * (int)kRec;
+  const unsigned sbc = ((unsigned)(g0w > 0 ? g0w - 1 : 0) * ntiles + tile) * (unsigned)nrow;  // seam rows read (row r at sbc + r - 1)
+  const unsigned sbp = ((unsigned)g0w * ntiles + tile) * (unsigned)nrow;                    // seam rows written
   {
     const int l = (int)(threadIdx.x & 63);
     long long stF, stS;
@@ -1277,7 +1282,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
       if (consumer) {
         // the flag of row rn was loaded one step ago; wait for it if the
         // producing strip is behind, then load the row's message
-        const size_t sr = sbc + (rn - 1);
+        const unsigned sr = sbc + (rn - 1);
         int fv = __builtin_amdgcn_readfirstlane(mflag);
         if (fv != epoch && __builtin_amdgcn_readfirstlane(ld_agent(werr)) == 0) {
           int it = 0;
@@ -1287,7 +1292,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
           } while (fv != epoch && ++it < kSpinMax);
           if (fv != epoch && lane == 0) st_agent(werr, 1);
         }
-        const char *m = reinterpret_cast<const char *>(msg + sr * 256) + loff;
+        const char *m = reinterpret_cast<const char *>(msg + (size_t)sr * 256) + loff;
         mfg = ld_agent(reinterpret_cast<const T *>(m));
         mdg = ld_agent(reinterpret_cast<const T *>(m + 64 * ES));
         mfh = ld_agent(reinterpret_cast<const T *>(m + 128 * ES));
@@ -1348,7 +1353,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
       bx_new = -negp(axr) * ra;
       by_new = -negp(ayt) * ra;
       if (producer) {
-        char *m = reinterpret_cast<char *>(msg + (sbp + (r - 1)) * 256) + loff;
+        char *m = reinterpret_cast<char *>(msg + (size_t)(sbp + (r - 1)) * 256) + loff;
         st_agent(reinterpret_cast<T *>(m), gfr);
         st_agent(reinterpret_cast<T *>(m + 64 * ES), gd_new);
         st_agent(reinterpret_cast<T *>(m + 128 * ES), hfr);
@@ -1356,7 +1361,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
         flag_due = 1;
       }
       if (consumer) {
-        T *m = dright + (sbc + (r - 1)) * 128;
+        T *m = dright + (size_t)(sbc + (r - 1)) * 128;
         stv(m, loff, gd_new);
         stv(m + 64, loff, hd_new);
       }
@@ -1921,6 +1926,9 @@ bool launch_strip_nw(const Dev &d, double *cpar, T *F, const T *slot, T *R, int 
   // kernel's register allocation is untouched by the WENO3 weights
   if (d.scheme == kSchemeWeno3)
     hipLaunchKernelGGL((k_strip<T, NW, 1>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cp, F, slot, R,
+                       store_rhs, ntiles, (T *)b.msg, (T *)b.dright, b.flag, b.epoch, b.err, b.oflags);
+  else if (d.scheme == 0)  // first-order upwind
+    hipLaunchKernelGGL((k_strip<T, NW, 2>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cp, F, slot, R,
                        store_rhs, ntiles, (T *)b.msg, (T *)b.dright, b.flag, b.epoch, b.err, b.oflags);
   else
     hipLaunchKernelGGL((k_strip<T, NW, 0>), dim3(ngroups * ntiles), dim3(64 * NW), 0, s, d, cp, F, slot, R,
