@@ -256,6 +256,30 @@ def test_backup_and_resume_continue_the_run(tmp_path):
     full.close()
 
 
+def test_nan_in_distribution_is_reported():
+    """A NaN in the distribution of one interior cell reaches the residual of
+    the next step, and gsis_residual stops with GSIS_ENAN, as the reference's
+    residual monitor stops on NaN (src/IO.f90:236-244); the oracle reports the
+    same."""
+    g = load_pkg()
+    case = g.cases.cavity_case(12, 16, kn=0.1, lid=0.15, dt=0.5)
+    gpu = solver_mod().GsisSolver(case, kstep_show=1)
+    cpu = load_oracle().OracleSolver(case)
+    for s in (gpu, cpu):
+        s.initialize()
+        f = s.frd()
+        f[0, 3 + 6, 3 + 6, 5, 7] = np.nan
+        s.upload(frd=f)
+        s.step(1)
+    with pytest.raises(g.capi.GsisError) as e:
+        gpu.residual()
+    assert e.value.code == g.capi.GSIS_ENAN
+    with pytest.raises(load_oracle().OracleError):
+        cpu.residual()
+    gpu.close()
+    cpu.close()
+
+
 def test_residual_cadence_matches_oracle():
     """With kstep_show=10 the RHS is kept only every 10th step; the residual
     history and the iteration count still match the oracle's."""
