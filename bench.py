@@ -47,12 +47,14 @@ def time_to_converge(pkg, solver, args, tol=1e-8, field_tol=1e-4, chunk=50, max_
     upwind (the reference's Iflag_Scheme=0): with the NND limiter the residual
     of these cases stalls at 1e-2..1 (limiter cycling, the reference's scheme
     as written), so no run reaches 1e-8.  Only the GPU stepping is timed (the
-    residual and field checks between chunks are not)."""
+    residual and field checks between chunks are not).  Also the FP64 time
+    to the same residual of the planar Couette configuration (4 x 128 cells,
+    32^2 Newton-Cotes velocities, Kn = 0.1, walls at -+0.1)."""
     case = make_case(pkg, args.n, args.nv, args.kn, args.lid, args.dt)
     case.const["scheme"] = pkg.capi.SCHEME_UP1
 
-    def run(prec, stop):
-        s = solver.GsisSolver(case, precision=prec, kstep_show=chunk)
+    def run(prec, stop, c=case):
+        s = solver.GsisSolver(c, precision=prec, kstep_show=chunk)
         s.initialize()
         s.counters()
         k, t = 0, 0.0
@@ -80,12 +82,18 @@ def time_to_converge(pkg, solver, args, tol=1e-8, field_tol=1e-4, chunk=50, max_
         return max(e.values()) <= field_tol, e
     k64f, t64f, e64f, _, ok64f = run(pkg.capi.PREC_FP64, fields_close)
     k32f, t32f, e32f, _, ok32f = run(pkg.capi.PREC_FP32, fields_close)
+    # the planar Couette configuration (4 x 128 cells, 32^2 velocities), same
+    # scheme, FP64 to the same residual
+    c1 = pkg.cases.couette_case(ny_cells=128, nx_cells=4, nv=32, kn=0.1, uwall=0.1, dt=1.0)
+    c1.const["scheme"] = pkg.capi.SCHEME_UP1
+    kc1, tc1, rc1, _, okc1 = run(pkg.capi.PREC_FP64, res_below, c1)
     return {
         "scheme": "UP1", "residual_tol": tol, "field_tol_rel_l2": field_tol, "check_every": chunk,
         "fp64_to_residual": {"steps": k64, "s": t64, "residual": r64, "reached": ok64},
         "fp64_to_field_tol": {"steps": k64f, "s": t64f, "errors": e64f, "reached": ok64f},
         "fp32_to_field_tol": {"steps": k32f, "s": t32f, "errors": e32f, "reached": ok32f},
         "fp32_field_tol_speedup_vs_fp64_converged": t64 / t32f if t32f > 0 else None,
+        "couette_4x128_v32_fp64_to_residual": {"steps": kc1, "s": tc1, "residual": rc1, "reached": okc1},
     }
 
 

@@ -82,6 +82,33 @@ def test_iteration_count_to_convergence(scheme):
     _compare(gpu, cpu, "cavity10 converged", frd=False)
 
 
+def test_couette_c1_iteration_count_to_1e8():
+    """The planar Couette configuration run to convergence: 4 x 128 cells
+    tanh-clustered, 32 x 32 Newton-Cotes velocities on [-6,6]^2, Kn = 0.1,
+    walls at -+0.1, from rest, first-order upwind (the NND residual of this
+    case stalls), dt = 1.  GPU and oracle reach all four residuals below 1e-8
+    at the same step, with the same residual history every 50 steps (to
+    1e-9 of the first residual: near convergence the residual is a small
+    difference of the fields, so its own relative error grows while the
+    fields still agree to 1e-10)."""
+    g = load_pkg()
+    case = g.cases.couette_case(ny_cells=128, nx_cells=4, nv=32, kn=0.1, uwall=0.1, dt=1.0)
+    case.const["scheme"] = g.capi.SCHEME_UP1
+    print("case:", case.name, "UP1 to 1e-8", flush=True)
+    gpu, cpu = _pair(case, kstep_show=50)
+    sg, hg = gpu.run(6000, tol=1e-8, show=50)
+    print("gpu steps", sg, hg[-1][1], flush=True)
+    sc, hc = cpu.run(6000, tol=1e-8, show=50)
+    print("oracle steps", sc, hc[-1][1], flush=True)
+    assert sg == sc and sg < 6000
+    scale = float(np.max(hc[0][1]))
+    for (ka, ra), (kb, rb) in zip(hg, hc):
+        assert ka == kb and float(np.max(np.abs(ra - rb))) <= 1e-9 * scale, (ka, ra, rb)
+    _compare(gpu, cpu, "couette C1 converged", frd=False)
+    gpu.close()
+    cpu.close()
+
+
 @pytest.mark.parametrize("maker", [channel_case, cylinder_case], ids=["channel50x6", "cylinder40"])
 def test_m1_shipped_parity(maker):
     """Mode M1 (synthetic NS solve on): same fields and inner iteration counts."""
