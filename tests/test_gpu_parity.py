@@ -46,13 +46,15 @@ def test_shipped_cases_step_parity(maker):
     cpu.close()
 
 
-def test_moving_lid_cavity_parity():
+@pytest.mark.parametrize("kn", [0.01, 0.1, 1.0])
+def test_moving_lid_cavity_parity(kn):
+    """The cavity at the three Knudsen numbers of the cavity configuration."""
     g = load_pkg()
-    case = g.cases.cavity_case(12, 24, kn=0.1, lid=0.15, dt=0.5)
+    case = g.cases.cavity_case(12, 24, kn=kn, lid=0.15, dt=0.5)
     gpu, cpu = _pair(case)
     gpu.step(20)
     cpu.step(20)
-    _compare(gpu, cpu, "cavity12 step 20")
+    _compare(gpu, cpu, f"cavity12 kn={kn} step 20")
     assert rel_l2(gpu.residual(), cpu.residual()) <= TOL
 
 
