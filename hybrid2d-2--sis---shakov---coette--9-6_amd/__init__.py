@@ -6,9 +6,12 @@ Submodules:
   cases   -- host-side producers: mesh geometry, velocity grids, constants,
              initial state, readers of the reference's input files
   solver  -- GsisSolver: the driver loop over the C-ABI
+  multiblock -- meshes of several blocks, their links and a driver with one
+             handle per block
+  restart -- restart files in the reference's layout
   fields  -- macroscopic fields and their relative L2 differences
   build   -- hipcc build of libgsis.so for gfx950 (and the flang host side)
 """
-from . import capi, cases, fields, restart  # noqa: F401
+from . import capi, cases, fields, multiblock, restart  # noqa: F401
 
 __all__ = ["capi", "cases", "fields"]

@@ -24,6 +24,7 @@ GSIS_ETEMP = 5
 BC_WALL = -2
 BC_FARFIELD = -4
 BC_PERIODIC = -5
+BC_SYMMETRY = -3  # passive in the kinetic step; a code >= 1 is an inner face (neighbour block number)
 
 SCHEME_UP1 = 0
 SCHEME_NND = 1
@@ -53,6 +54,7 @@ EXPORTED = (
     "gsis_download_frd",
     "gsis_counters",
     "gsis_set_counters",
+    "gsis_inner_exchange",
     "gsis_timing",
     "gsis_destroy",
     "gsis_last_error",
@@ -155,6 +157,7 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.gsis_download_frd.argtypes = [h, c_dp]
     lib.gsis_counters.argtypes = [h, ctypes.POINTER(ctypes.c_int32), c_dp, ctypes.POINTER(ctypes.c_int32)]
     lib.gsis_set_counters.argtypes = [h, ctypes.c_int32, ctypes.c_double]
+    lib.gsis_inner_exchange.argtypes = [h, ctypes.c_int32, h, ctypes.c_int32, ctypes.c_int32]
     lib.gsis_timing.argtypes = [h, ctypes.c_int32, c_dp, ctypes.POINTER(ctypes.c_int32)]
     lib.gsis_destroy.argtypes = [h]
     lib.gsis_last_error.restype = ctypes.c_char_p

@@ -180,10 +180,15 @@ class Mesh:
         return x1, y1
 
 
-def build_mesh(xn: np.ndarray, yn: np.ndarray, bcs: Sequence["BC"], lref: float = 1.0) -> Mesh:
+def build_mesh(xn: np.ndarray, yn: np.ndarray, bcs: Sequence["BC"], lref: float = 1.0,
+               inner: Optional[dict] = None) -> Mesh:
     """Geometry of Creat_Mesh (src/sub_init.f90:178-208) and the ghost nodes of
-    Update_coordinate_buffer_onemesh for physical boundaries + corners
-    (src/sub_init.f90:777-810).  xn, yn: node coordinates (nx, ny)."""
+    Update_coordinate_buffer_onemesh (src/sub_init.f90:719-823): linear
+    extrapolation on physical faces, then the corners.  xn, yn: node
+    coordinates (nx, ny).  ``inner`` maps the list index of an inner subface
+    to (xn_nb, yn_nb, bc_nb, orient): the neighbour block's nodes, its matching
+    subface and the orientation; that face's ghost nodes are the neighbour's
+    nodes next to bc_nb instead (:739-770)."""
     nx, ny = xn.shape
     x = np.zeros((nx + 4, ny + 4), order="F")
     y = np.zeros((nx + 4, ny + 4), order="F")
@@ -217,9 +222,35 @@ def build_mesh(xn: np.ndarray, yn: np.ndarray, bcs: Sequence["BC"], lref: float 
             sj[i - 1, j - 1] = s
             nj1[i - 1, j - 1] = -dy / s
             nj2[i - 1, j - 1] = dx / s
-    # ghost nodes of physical boundaries: linear extrapolation, in BC order
-    for b in bcs:
+    # ghost nodes: neighbour nodes on inner faces, linear extrapolation on
+    # physical ones, in BC order
+    for kb, b in enumerate(bcs):
         ib, ie, jb, je = b.ist, b.iend, b.jst, b.jend
+        if inner is not None and kb in inner:
+            xnb, ynb, c, orient = inner[kb]
+            n1 = ie - ib + je - jb + 1
+            for l in range(1, LAP + 1):
+                for k in range(1, n1 + 1):
+                    m = k if orient == 2 else n1 + 1 - k  # position along the neighbour's face
+                    if c.face == 1:
+                        sa = (c.ist + l, c.jst + m - 1)
+                    elif c.face == 2:
+                        sa = (c.ist + m - 1, c.jst + l)
+                    elif c.face == 3:
+                        sa = (c.iend - l, c.jst + m - 1)
+                    else:
+                        sa = (c.ist + m - 1, c.jend - l)
+                    if b.face == 1:
+                        da = (ib - l, jb + k - 1)
+                    elif b.face == 2:
+                        da = (ib + k - 1, jb - l)
+                    elif b.face == 3:
+                        da = (ie + l, jb + k - 1)
+                    else:
+                        da = (ib + k - 1, je + l)
+                    x[da[0] + o, da[1] + o] = xnb[sa[0] - 1, sa[1] - 1] / lref
+                    y[da[0] + o, da[1] + o] = ynb[sa[0] - 1, sa[1] - 1] / lref
+            continue
         for k in range(1, LAP + 1):
             if b.face == 1:
                 for a in (x, y):
@@ -251,9 +282,12 @@ class BC:
     iend: int
     jst: int
     jend: int
-    code: int
+    code: int  # GSIS_BC_* or, for an inner face, the neighbour block number (>= 1)
     uw: float = 0.0
     vw: float = 0.0
+    # inner face: the neighbour's node range (ist, iend, jst, jend) as given on
+    # the neighbour line of generic.inp (src/sub_init.f90:282-284)
+    nrange: Optional[tuple] = None
 
 
 def bc_from_range(ist: int, iend: int, jst: int, jend: int, code_pos: int, nx: int, ny: int,
@@ -275,23 +309,62 @@ def bc_from_range(ist: int, iend: int, jst: int, jend: int, code_pos: int, nx: i
 # ---------------------------------------------------------------------------
 # reference input files
 # ---------------------------------------------------------------------------
-def _tokens(text: str):
-    for tok in text.replace(",", " ").split():
-        yield tok
+class ListDirected:
+    """Fortran list-directed READ over the records (lines) of a text file: each
+    READ starts at the next record, takes values across records until it has
+    what it asks for and drops the rest of its last record.  Separators are
+    blanks and commas; ``r*c`` repeats c r times."""
+
+    def __init__(self, text: str):
+        self.lines = text.splitlines()
+        self.pos = 0
+
+    def read(self, n: int) -> List[str]:
+        out: List[str] = []
+        while len(out) < n:
+            if self.pos >= len(self.lines):
+                raise EOFError("end of file in a list-directed read")
+            for tok in self.lines[self.pos].replace(",", " ").split():
+                if "*" in tok:
+                    r, c = tok.split("*", 1)
+                    out.extend([c] * int(r))
+                else:
+                    out.append(tok)
+            self.pos += 1
+        return out[:n]
+
+    def skip(self) -> None:
+        """A READ with an empty list: skips one record."""
+        self.pos += 1
+
+
+def _real(tok: str) -> float:
+    return float(tok.replace("d", "e").replace("D", "e"))
+
+
+def read_generic_dat_blocks(path: str | pathlib.Path) -> List[tuple]:
+    """generic.dat: NB, then per block a record with NI NJ (anything after them
+    on that record, such as NK, is dropped) and the list x, y, z of NI*NJ
+    values each (src/sub_init.f90:103-178).  Returns [(xn, yn)] with node
+    arrays of shape (NI, NJ)."""
+    f = ListDirected(pathlib.Path(path).read_text())
+    nb = int(f.read(1)[0])
+    out = []
+    for _ in range(nb):
+        ni, nj = (int(t) for t in f.read(2))
+        vals = np.array([_real(t) for t in f.read(3 * ni * nj)])
+        xn = vals[:ni * nj].reshape((ni, nj), order="F")
+        yn = vals[ni * nj:2 * ni * nj].reshape((ni, nj), order="F")
+        out.append((xn, yn))
+    return out
 
 
 def read_generic_dat(path: str | pathlib.Path):
-    """generic.dat: NB; NI NJ NK; x, y, z (list-directed), src/sub_init.f90:102-178."""
-    toks = list(_tokens(pathlib.Path(path).read_text()))
-    nb = int(toks[0])
-    if nb != 1:
-        raise ValueError("only single-block meshes are supported")
-    ni, nj = int(toks[1]), int(toks[2])
-    pos = 4 if len(toks) > 3 + 3 * ni * nj else 3
-    vals = np.array([float(t.replace("d", "e").replace("D", "e")) for t in toks[pos:pos + 2 * ni * nj]])
-    xn = vals[:ni * nj].reshape((ni, nj), order="F")
-    yn = vals[ni * nj:2 * ni * nj].reshape((ni, nj), order="F")
-    return xn, yn
+    """generic.dat of one block: (xn, yn)."""
+    blocks = read_generic_dat_blocks(path)
+    if len(blocks) != 1:
+        raise ValueError("only single-block meshes are supported here (see multiblock)")
+    return blocks[0]
 
 
 def read_generic_inp(path: str | pathlib.Path, nx: int, ny: int) -> List[BC]:
@@ -699,8 +772,9 @@ def initial_state(case: Case):
 def ns_boundary_init(case: Case, u: np.ndarray, swall: np.ndarray) -> np.ndarray:
     """NS ghost cells at initialisation, Boundary_condition_onemesh
     (src/sub_boundary.f90:10-39): far field = isothermal wall fed by the
-    kinetic wall moments Swall (boundary_Farfield GKUA branch, :204-229) and
-    periodic "inlet" (:248-298).  ``u`` is (4,-1:nx+1,-1:ny+1) Fortran."""
+    kinetic wall moments Swall (boundary_Farfield GKUA branch, :204-229),
+    periodic "inlet" (:248-298) and the symmetry mirror
+    (Symmetry_or_slidewall, :410-471).  ``u`` is (4,-1:nx+1,-1:ny+1) Fortran."""
     c = case.const
     g, ma = c["gamma"], c["ma"]
     nx, ny = case.nx, case.ny
@@ -763,6 +837,36 @@ def ns_boundary_init(case: Case, u: np.ndarray, swall: np.ndarray) -> np.ndarray
                         else:
                             i1, j1, i2, j2 = i, k, i, j + k - 1
                         u[:, i2 + 1, j2 + 1] = u[:, i1 + 1, j1 + 1]
+        elif b.code == capi.BC_SYMMETRY:
+            # every node of the range, the last one included; the face normal
+            # at a node comes from the edge to the next node (:433-444)
+            xg, yg = case.mesh.x, case.mesh.y
+            for j in range(jb, je + 1):
+                for i in range(ib, ie + 1):
+                    if b.face in (1, 3):
+                        dx = xg[i + 1, j + 2] - xg[i + 1, j + 1]
+                        dy = yg[i + 1, j + 2] - yg[i + 1, j + 1]
+                        si = math.sqrt(dx * dx + dy * dy)
+                        n1, n2 = dy / si, -dx / si
+                    else:
+                        dx = xg[i + 2, j + 1] - xg[i + 1, j + 1]
+                        dy = yg[i + 2, j + 1] - yg[i + 1, j + 1]
+                        si = math.sqrt(dx * dx + dy * dy)
+                        n1, n2 = -dy / si, dx / si
+                    for k in range(1, LAP + 1):
+                        if b.face == 1:
+                            i1, j1, i2, j2 = i + k - 1, j, i - k, j
+                        elif b.face == 2:
+                            i1, j1, i2, j2 = i, j + k - 1, i, j - k
+                        elif b.face == 3:
+                            i1, j1, i2, j2 = i - k, j, i + k - 1, j
+                        else:
+                            i1, j1, i2, j2 = i, j - k, i, j + k - 1
+                        vn = U(1, i1, j1) * n1 + U(2, i1, j1) * n2
+                        u[0, i2 + 1, j2 + 1] = U(0, i1, j1)
+                        u[1, i2 + 1, j2 + 1] = U(1, i1, j1) - 2.0 * vn * n1
+                        u[2, i2 + 1, j2 + 1] = U(2, i1, j1) - 2.0 * vn * n2
+                        u[3, i2 + 1, j2 + 1] = U(3, i1, j1)
         if b.code in (capi.BC_WALL, capi.BC_FARFIELD):
             soff += 10 * nk
     return u

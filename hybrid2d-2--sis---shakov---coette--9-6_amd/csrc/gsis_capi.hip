@@ -42,6 +42,7 @@ struct gsis_handle {
   std::vector<BcDev> bcd;
   Dev d{};
   hipStream_t s = nullptr;
+  int dev = 0;  // device current at gsis_create
   int prec = 0;
   size_t tsize = 8;
   void *FG = nullptr, *FH = nullptr, *slot = nullptr, *RG = nullptr, *RH = nullptr;
@@ -110,8 +111,12 @@ int validate(const gsis_params *p) {
     if (!ok) return fail(GSIS_EARG, "BC " + std::to_string(b) + ": face does not match its index range");
     if (c.ist < 1 || c.iend > p->nx || c.jst < 1 || c.jend > p->ny || c.ist > c.iend || c.jst > c.jend)
       return fail(GSIS_EARG, "BC " + std::to_string(b) + ": index range outside the block");
-    if (c.code != GSIS_BC_WALL && c.code != GSIS_BC_FARFIELD && c.code != GSIS_BC_PERIODIC)
-      return fail(GSIS_EUNSUPPORTED, "BC " + std::to_string(b) + ": only wall, farfield and periodic are implemented");
+    // symmetry and inner faces are passive in the kinetic step: no ghost fill
+    // (src/GKUA_boundary.f90:25-35) and no face override (src/GKUA_Residual.f90:189-192)
+    if (c.code != GSIS_BC_WALL && c.code != GSIS_BC_FARFIELD && c.code != GSIS_BC_PERIODIC &&
+        c.code != GSIS_BC_SYMMETRY && c.code < 1)
+      return fail(GSIS_EUNSUPPORTED, "BC " + std::to_string(b) +
+                                         ": only wall, farfield, periodic, symmetry and inner faces are implemented");
   }
   return 0;
 }
@@ -260,6 +265,7 @@ int gsis_create(const gsis_params *p, gsis_handle **out) {
     gsis_destroy(h);
     return code;
   };
+  if (hipGetDevice(&h->dev) != hipSuccess) return cleanup(fail(GSIS_EHIP, "hipGetDevice failed"));
   if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(GSIS_EHIP, "hipStreamCreate failed"));
   // velocity grid
@@ -585,6 +591,46 @@ int gsis_counters(gsis_handle *h, int32_t *kstep, double *tt, int32_t *ns_iter) 
   if (kstep) *kstep = h->kstep;
   if (tt) *tt = h->tt;
   if (ns_iter) *ns_iter = h->ns_iter;
+  return 0;
+}
+
+int gsis_inner_exchange(gsis_handle *dst, int32_t ksub, gsis_handle *src, int32_t msub, int32_t orient) {
+  if (!dst || !src) return fail(GSIS_EARG, "NULL handle");
+  if (ksub < 1 || ksub > (int32_t)dst->bcd.size() || msub < 1 || msub > (int32_t)src->bcd.size())
+    return fail(GSIS_EARG, "subface number out of range");
+  const BcDev &bd = dst->bcd[ksub - 1], &bs = src->bcd[msub - 1];
+  if (bd.code < 1) return fail(GSIS_EARG, "subface ksub of dst is not an inner face (code >= 1)");
+  if (dst->prec != src->prec) return fail(GSIS_EARG, "the two blocks have different precisions");
+  if (dst->d.nvi != src->d.nvi || dst->d.nvj != src->d.nvj)
+    return fail(GSIS_EARG, "the two blocks have different velocity grids");
+  if (dst->dev != src->dev) return fail(GSIS_EARG, "the two blocks live on different devices");
+  // n1 cells along each face (src/GKUA_boundary.f90:427); the reference's
+  // buffer is sized by this block's face, so the neighbour's must match
+  const int n1 = bd.iend - bd.ist + bd.jend - bd.jst, n1s = bs.iend - bs.ist + bs.jend - bs.jst;
+  if (n1 < 1 || n1 != n1s) return fail(GSIS_EARG, "the two subfaces have different cell counts");
+  auto launch = [&]() {
+    if (dst->prec == GSIS_PREC_FP32)
+      gsis::launch_inner<float>(dst->d, bd, (float *)dst->FG, (float *)dst->FH, src->d, bs, (const float *)src->FG,
+                                (const float *)src->FH, n1, orient, dst->s);
+    else
+      gsis::launch_inner<double>(dst->d, bd, (double *)dst->FG, (double *)dst->FH, src->d, bs,
+                                 (const double *)src->FG, (const double *)src->FH, n1, orient, dst->s);
+    return hipGetLastError();
+  };
+  if (src == dst) {
+    HIPCHK(launch());
+    return 0;
+  }
+  // two streams: the copy follows src's queued work, and src's later work follows the copy
+  hipEvent_t e = nullptr;
+  HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  hipError_t r = hipEventRecord(e, src->s);
+  if (r == hipSuccess) r = hipStreamWaitEvent(dst->s, e, 0);
+  if (r == hipSuccess) r = launch();
+  if (r == hipSuccess) r = hipEventRecord(e, dst->s);
+  if (r == hipSuccess) r = hipStreamWaitEvent(src->s, e, 0);
+  (void)hipEventDestroy(e);
+  if (r != hipSuccess) return fail(GSIS_EHIP, std::string("inner_exchange: ") + hipGetErrorString(r));
   return 0;
 }
 

@@ -18,6 +18,10 @@ struct WallSet {
 template <typename T> void launch_walls(const Dev &d, const WallSet &ws, T *FG, T *FH, T *slot, hipStream_t s);
 template <typename T> void launch_farfield(const Dev &d, const BcDev &b, T *FG, T *FH, hipStream_t s);
 template <typename T> void launch_periodic(const Dev &d, const BcDev &b, T *FG, T *FH, hipStream_t s);
+// inner-face exchange of LAP = 2 layers, FRD(1:2) and S_GKUA(1:10); n1 cells along the face
+template <typename T>
+void launch_inner(const Dev &dd, const BcDev &bd, T *FGd, T *FHd, const Dev &ds, const BcDev &bs, const T *FGs,
+                  const T *FHs, int n1, int orient, hipStream_t s);
 template <typename T> void launch_rhs(const Dev &d, const T *FG, const T *FH, const T *slot, T *RG, T *RH, hipStream_t s);
 template <typename T> void launch_sweep(const Dev &d, T *FG, T *FH, const T *RG, const T *RH, hipStream_t s);
 template <typename T> void launch_moments(const Dev &d, T *FG, T *FH, hipStream_t s);

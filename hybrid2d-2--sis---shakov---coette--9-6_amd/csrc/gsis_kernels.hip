@@ -349,6 +349,46 @@ __global__ void k_periodic(Dev d, BcDev b, T *FG, T *FH) {
 }
 
 // ------------------------------------------------------------------------
+// Inner face between two blocks: GKUA_inner_mesco_boundary and
+// GKUA_inner_macro_boundary (src/GKUA_boundary.f90:400-633).  The LAP = 2 cell
+// layers next to the neighbour's subface are copied into this block's ghost
+// layers: FRD(1:2) for every velocity and S_GKUA(1:10).  Position k = 1..n1
+// along this block's face takes the neighbour's position k (orient == 2) or
+// n1+1-k (any other orient).  One workgroup per (position, layer); threads
+// stride over velocities.  The neighbour's cells are interior cells and this
+// block's are ghost cells, so a block may be its own neighbour.
+// ------------------------------------------------------------------------
+__host__ __device__ inline void inner_src_cell(const BcDev &b, int m, int l, int &i, int &j) {
+  if (b.face == 1) { i = b.ist + l - 1; j = b.jst + m - 1; }
+  else if (b.face == 2) { i = b.ist + m - 1; j = b.jst + l - 1; }
+  else if (b.face == 3) { i = b.iend - l; j = b.jst + m - 1; }
+  else { i = b.ist + m - 1; j = b.jend - l; }
+}
+__host__ __device__ inline void inner_ghost_cell(const BcDev &b, int k, int l, int &i, int &j) {
+  if (b.face == 1) { i = b.ist - l; j = b.jst + k - 1; }
+  else if (b.face == 2) { i = b.ist + k - 1; j = b.jst - l; }
+  else if (b.face == 3) { i = b.iend + l - 1; j = b.jst + k - 1; }
+  else { i = b.ist + k - 1; j = b.jend + l - 1; }
+}
+
+template <typename T>
+__global__ void k_inner(Dev dd, BcDev bd, T *FGd, T *FHd, Dev ds, BcDev bs, const T *FGs, const T *FHs, int n1,
+                        int orient) {
+  const int k = blockIdx.x % n1 + 1, l = blockIdx.x / n1 + 1;
+  const int k1 = orient == 2 ? k : n1 + 1 - k;
+  int ia, ja, ib, jb;
+  inner_src_cell(bs, k1, l, ia, ja);
+  inner_ghost_cell(bd, k, l, ib, jb);
+  const int ca = cidx(ds, ia, ja), cb = cidx(dd, ib, jb);
+  if (threadIdx.x < 10) dd.S[threadIdx.x * dd.ncell + cb] = ds.S[threadIdx.x * ds.ncell + ca];
+  const size_t sa = (size_t)ca * ds.nv, sb = (size_t)cb * dd.nv;
+  for (int v = threadIdx.x; v < dd.nv; v += blockDim.x) {
+    FGd[sb + v] = FGs[sa + v];
+    FHd[sb + v] = FHs[sa + v];
+  }
+}
+
+// ------------------------------------------------------------------------
 // Kinetic RHS for every interior cell and velocity: reduced Shakhov
 // equilibrium (Get_Feq, src/GKUA_Residual.f90:420-469), NND upwind fluxes with
 // the wall/farfield face overrides and the BGK-Shakhov source (RHSFDVsub,
@@ -1843,6 +1883,11 @@ void launch_periodic(const Dev &d, const BcDev &b, T *FG, T *FH, hipStream_t s) 
   hipLaunchKernelGGL(k_periodic<T>, dim3(2 * nn), dim3(256), 0, s, d, b, FG, FH);
 }
 template <typename T>
+void launch_inner(const Dev &dd, const BcDev &bd, T *FGd, T *FHd, const Dev &ds, const BcDev &bs, const T *FGs,
+                  const T *FHs, int n1, int orient, hipStream_t s) {
+  hipLaunchKernelGGL(k_inner<T>, dim3(2 * n1), dim3(256), 0, s, dd, bd, FGd, FHd, ds, bs, FGs, FHs, n1, orient);
+}
+template <typename T>
 void launch_rhs(const Dev &d, const T *FG, const T *FH, const T *slot, T *RG, T *RH, hipStream_t s) {
   hipLaunchKernelGGL(k_rhs<T>, dim3((d.nx - 1) * (d.ny - 1)), dim3(256), 0, s, d, FG, FH, slot, RG, RH);
 }
@@ -2019,6 +2064,8 @@ void launch_residual(const Dev &d, double *out, hipStream_t s) {
   template void launch_walls<T>(const Dev &, const WallSet &, T *, T *, T *, hipStream_t);        \
   template void launch_farfield<T>(const Dev &, const BcDev &, T *, T *, hipStream_t);            \
   template void launch_periodic<T>(const Dev &, const BcDev &, T *, T *, hipStream_t);            \
+  template void launch_inner<T>(const Dev &, const BcDev &, T *, T *, const Dev &, const BcDev &, const T *, \
+                                const T *, int, int, hipStream_t);                                \
   template void launch_rhs<T>(const Dev &, const T *, const T *, const T *, T *, T *, hipStream_t); \
   template void launch_sweep<T>(const Dev &, T *, T *, const T *, const T *, hipStream_t);        \
   template void launch_moments<T>(const Dev &, T *, T *, hipStream_t);                            \

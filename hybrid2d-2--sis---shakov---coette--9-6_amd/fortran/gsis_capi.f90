@@ -15,6 +15,8 @@ module gsis_capi
   integer(c_int), parameter :: GSIS_OK = 0, GSIS_EARG = 1, GSIS_EHIP = 2, GSIS_ENAN = 3, &
                                GSIS_EUNSUPPORTED = 4, GSIS_ETEMP = 5
   integer(c_int32_t), parameter :: GSIS_BC_WALL = -2, GSIS_BC_FARFIELD = -4, GSIS_BC_PERIODIC = -5
+  ! symmetry face; a code >= 1 is an inner face (the neighbour block number)
+  integer(c_int32_t), parameter :: GSIS_BC_SYMMETRY = -3
   integer(c_int32_t), parameter :: GSIS_SCHEME_UP1 = 0, GSIS_SCHEME_NND = 1, GSIS_SCHEME_WENO3 = 3
   integer(c_int32_t), parameter :: GSIS_MODE_M0 = 0, GSIS_MODE_M1 = 1
   integer(c_int32_t), parameter :: GSIS_PREC_FP64 = 0, GSIS_PREC_FP32 = 1
@@ -97,6 +99,14 @@ module gsis_capi
       type(c_ptr), value :: h
       integer(c_int32_t), value :: kstep
       real(c_double), value :: tt
+    end function
+    ! GKUA_inner_mesco_boundary + GKUA_inner_macro_boundary for one inner
+    ! subface (src/GKUA_boundary.f90:400-633); ksub, msub 1-based, orient = Bc%orient
+    integer(c_int) function gsis_inner_exchange(dst, ksub, src, msub, orient) &
+        bind(C, name="gsis_inner_exchange")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: dst, src
+      integer(c_int32_t), value :: ksub, msub, orient
     end function
     integer(c_int) function gsis_destroy(h) bind(C, name="gsis_destroy")
       import :: c_int, c_ptr

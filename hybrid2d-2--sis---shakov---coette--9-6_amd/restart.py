@@ -5,10 +5,10 @@ The Fortran host keeps its own backup_Mesco_write / backup_Mesco_read
 (src/Backup_write_read.f90:150-209, 56-112) and only needs gsis_download_frd /
 gsis_upload_frd around them.  This module writes and reads the same bytes:
 
-* ``restart{1,2}/star00/restart00000.dat``: one Fortran unformatted sequential
-  record holding B%FRD(2,-3:nx+3,-3:ny+3,nvi,nvj) as REAL(8), the file of MPI
-  rank 0, the only rank of a single-GPU run (the file names follow
-  :185-193 with myid = 0);
+* ``restart{1,2}/star00/restart00000.dat``: per block, in block order, one
+  Fortran unformatted sequential record holding B%FRD(2,-3:nx+3,-3:ny+3,nvi,nvj)
+  as REAL(8) (:201-206, :95-101); the file of MPI rank 0, the only rank of a
+  single-GPU run (the file names follow :185-193 with myid = 0);
 * ``restart{1,2}/flag.dat``: list-directed 1, Mesh(1)%tt, Mesh(1)%Kstep
   (backup_Macro_write, :114-147).
 
@@ -74,14 +74,17 @@ def read_record(f) -> bytes:
             return b"".join(parts)
 
 
-def write_restart(root: str | pathlib.Path, flag: int, frd: np.ndarray, tt: float, kstep: int,
+def write_restart(root: str | pathlib.Path, flag: int, frd, tt: float, kstep: int,
                   max_sub: int = MAX_SUBRECORD) -> pathlib.Path:
-    """backup_Macro_write(flag): the distribution first, then flag.dat."""
+    """backup_Macro_write(flag): the distribution first, then flag.dat.
+    ``frd`` is one block's array or a list of them, one record each."""
     d = pathlib.Path(root) / backup_dir(flag)
     (d / "star00").mkdir(parents=True, exist_ok=True)
-    a = np.asfortranarray(frd, dtype="<f8")
+    blocks = frd if isinstance(frd, (list, tuple)) else [frd]
     with open(d / "star00" / "restart00000.dat", "wb") as f:
-        write_record(f, a.ravel(order="F").data, max_sub)
+        for one in blocks:
+            a = np.asfortranarray(one, dtype="<f8")
+            write_record(f, a.ravel(order="F").data, max_sub)
     (d / "flag.dat").write_text(f"{1:12d}\n{float(tt)!r}\n{int(kstep):12d}\n")
     return d
 
@@ -94,18 +97,23 @@ def _flag(path: pathlib.Path):
     return int(tok[0]), float(tok[1].replace("D", "E").replace("d", "e")), int(tok[2])
 
 
-def read_restart(root: str | pathlib.Path, shape) -> tuple[np.ndarray, float, int, int]:
+def read_restart(root: str | pathlib.Path, shape) -> tuple:
     """init_flow_read + backup_Mesco_read: (frd, tt, kstep, slot) from the slot
-    whose flag is 1, restart2 first.  ``shape`` is FRD's (2, nx+7, ny+7, nvi, nvj)."""
+    whose flag is 1, restart2 first.  ``shape`` is FRD's (2, nx+7, ny+7, nvi, nvj),
+    or a list of such shapes, one per block, for which frd is a list."""
     root = pathlib.Path(root)
+    multi = len(shape) > 0 and isinstance(shape[0], (list, tuple))
+    shapes = list(shape) if multi else [shape]
     for slot in (2, 1):
         fl = _flag(root / backup_dir(slot) / "flag.dat")
         if fl is not None and fl[0] == 1:
+            frds = []
             with open(root / backup_dir(slot) / "star00" / "restart00000.dat", "rb") as f:
-                raw = read_record(f)
-            n = int(np.prod(shape))
-            if len(raw) != 8 * n:
-                raise ValueError(f"restart record holds {len(raw)} bytes, FRD needs {8 * n}")
-            frd = np.frombuffer(raw, dtype="<f8").reshape(shape, order="F").copy()
-            return frd, fl[1], fl[2], slot
+                for m, shp in enumerate(shapes, start=1):
+                    raw = read_record(f)
+                    n = int(np.prod(shp))
+                    if len(raw) != 8 * n:
+                        raise ValueError(f"restart record {m} holds {len(raw)} bytes, FRD needs {8 * n}")
+                    frds.append(np.frombuffer(raw, dtype="<f8").reshape(shp, order="F").copy())
+            return (frds if multi else frds[0]), fl[1], fl[2], slot
     raise FileNotFoundError("No backup data is available")

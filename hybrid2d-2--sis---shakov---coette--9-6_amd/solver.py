@@ -89,8 +89,17 @@ class GsisSolver:
         frd, tt, kstep, slot = restart.read_restart(root, frd_shape(self.case))
         self.upload(frd=frd)
         self.refresh_moments()
-        capi.check(self.lib, self.lib.gsis_set_counters(self.h, int(kstep), float(tt)))
+        self.set_counters(kstep, tt)
         return slot
+
+    def set_counters(self, kstep: int, tt: float) -> None:
+        capi.check(self.lib, self.lib.gsis_set_counters(self.h, int(kstep), float(tt)))
+
+    # -- inner faces between blocks (src/GKUA_boundary.f90:400-633) --------
+    def inner_exchange(self, ksub: int, src: "GsisSolver", msub: int, orient: int) -> None:
+        """Fill the ghost layers of subface ksub (1-based) from the cells next to
+        subface msub of ``src`` (another handle, or this one)."""
+        capi.check(self.lib, self.lib.gsis_inner_exchange(self.h, int(ksub), src.h, int(msub), int(orient)))
 
     # -- stepping ---------------------------------------------------------
     def step(self, n: int = 1):

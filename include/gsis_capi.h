@@ -40,6 +40,16 @@ extern "C" {
 #define GSIS_BC_WALL (-2)
 #define GSIS_BC_FARFIELD (-4)
 #define GSIS_BC_PERIODIC (-5) /* "inlet": periodic copy, src/GKUA_boundary.f90:219-273 */
+#define GSIS_BC_SYMMETRY (-3) /* symmetry or slide wall.  The kinetic step neither fills its
+                                 ghost cells (src/GKUA_boundary.f90:25-35) nor overrides its
+                                 fluxes (src/GKUA_Residual.f90:189-192): the ghost cells keep
+                                 what was uploaded.  Its only other effect in the reference is
+                                 velocity-space bookkeeping (src/Symetric_X.f90:3-46,
+                                 src/sub_init.f90:279-281,480-533). */
+/* A code >= 1 is an inner face: the number of the neighbour block, as set from
+ * the neighbour line of generic.inp (src/sub_init.f90:282-284).  The kinetic step
+ * treats it like GSIS_BC_SYMMETRY; its ghost cells change only through
+ * gsis_inner_exchange. */
 
 /* kinetic flux schemes (src/COMMONDATA.f90:18) */
 #define GSIS_SCHEME_UP1 0
@@ -167,6 +177,20 @@ int gsis_counters(gsis_handle *h, int32_t *kstep, double *tt, int32_t *ns_iter);
  * (backup_Mesco_read, :93-109).  The residual cadence (kstep_show) follows the
  * restored count.  kstep must be >= 0. */
 int gsis_set_counters(gsis_handle *h, int32_t kstep, double tt);
+
+/* GKUA_inner_mesco_boundary + GKUA_inner_macro_boundary (src/GKUA_boundary.f90:400-633)
+ * for one inner subface: the LAP = 2 cell layers next to subface msub of block
+ * src are copied into the ghost layers of subface ksub of block dst, FRD(1:2)
+ * for every velocity and S_GKUA(1:10).  ksub and msub are 1-based positions in
+ * each handle's BC list; ksub's code must be >= 1.  orient == 2 keeps the
+ * direction along the face, any other value reverses it (Bc%orient from
+ * Get_neighb_info, src/sub_init.f90:636-693).  Both faces must have the same
+ * cell count, and both handles the same precision, velocity grid and device;
+ * src may equal dst.  Asynchronous, on dst's stream, ordered after the work
+ * already queued on src's stream and before src's later work.  The reference
+ * runs it when a restart is read (src/Backup_write_read.f90:106-107), after the
+ * distribution of every block is in place. */
+int gsis_inner_exchange(gsis_handle *dst, int32_t ksub, gsis_handle *src, int32_t msub, int32_t orient);
 
 /* Kernel timing with HIP events on the handle's stream.  No counterpart in
  * the reference: it exists for benchmarks.  enable=1 clears and starts the

@@ -79,6 +79,53 @@ def van_leer(ql, qr, gamma):
     return f
 
 
+# ---------------------------------------------------------------------------
+# inner faces between blocks, on Fortran-layout arrays
+# ---------------------------------------------------------------------------
+LAP = 2  # COMMONDATA.f90:20
+
+
+def _inner_src_cell(b, m: int, l: int):
+    """Neighbour cell that fills buffer position m (1..n1), layer l (1..LAP):
+    the Ua fill of GKUA_inner_mesco_boundary (src/GKUA_boundary.f90:437-467)."""
+    if b.face == 1:
+        return b.ist + l - 1, b.jst + m - 1
+    if b.face == 2:
+        return b.ist + m - 1, b.jst + l - 1
+    if b.face == 3:
+        return b.iend - l, b.jst + m - 1
+    return b.ist + m - 1, b.jend - l
+
+
+def _inner_ghost_cell(b, k: int, l: int):
+    """Ghost cell written from buffer position k, layer l (src/GKUA_boundary.f90:470-494)."""
+    if b.face == 1:
+        return b.ist - l, b.jst + k - 1
+    if b.face == 2:
+        return b.ist + k - 1, b.jst - l
+    if b.face == 3:
+        return b.iend + l - 1, b.jst + k - 1
+    return b.ist + k - 1, b.jend + l - 1
+
+
+def inner_exchange_arrays(frd_d, s_d, bd, frd_s, s_s, bs, orient: int) -> None:
+    """GKUA_inner_mesco_boundary + GKUA_inner_macro_boundary
+    (src/GKUA_boundary.f90:400-633) for one subface, in place on frd_d, s_d:
+    FRD(2,-3:nx+3,-3:ny+3,nvi,nvj) and S_GKUA(11,-3:nx+3,-3:ny+3) of the
+    receiving block from those of the neighbour.  n1 is the receiving face's
+    cell count; orient == 2 keeps the direction, anything else reverses it."""
+    n1 = bd.iend - bd.ist + bd.jend - bd.jst
+    if n1 != bs.iend - bs.ist + bs.jend - bs.jst:
+        raise ValueError("the two subfaces have different cell counts")
+    for k in range(1, n1 + 1):
+        k1 = k if orient == 2 else n1 + 1 - k
+        for l in range(1, LAP + 1):
+            ia, ja = _inner_src_cell(bs, k1, l)
+            ib, jb = _inner_ghost_cell(bd, k, l)
+            frd_d[:, ib + 3, jb + 3] = frd_s[:, ia + 3, ja + 3]
+            s_d[0:10, ib + 3, jb + 3] = s_s[0:10, ia + 3, ja + 3]
+
+
 class OracleError(RuntimeError):
     pass
 
@@ -158,6 +205,18 @@ class OracleSolver:
         a = np.zeros((5, c.nx + 3, c.ny + 3), order="F")
         self.lib.orc_get_prims(self.h, _dp(a))
         return a
+
+    def inner_exchange(self, ksub: int, src: "OracleSolver", msub: int, orient: int) -> None:
+        """The exchange of subface ksub (1-based) from subface msub of ``src``."""
+        if (src.case.nvi, src.case.nvj) != (self.case.nvi, self.case.nvj):
+            raise ValueError("the two blocks have different velocity grids")
+        frd_d, s_d = self.frd(), self.macro()["s"]
+        if src is self:
+            frd_s, s_s = frd_d, s_d
+        else:
+            frd_s, s_s = src.frd(), src.macro()["s"]
+        inner_exchange_arrays(frd_d, s_d, self.case.bcs[ksub - 1], frd_s, s_s, src.case.bcs[msub - 1], orient)
+        self.upload(frd=frd_d, s=s_d)
 
     def convert_ns(self) -> int:
         return self.lib.orc_convert_ns(self.h)
