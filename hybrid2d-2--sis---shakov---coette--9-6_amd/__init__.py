@@ -9,9 +9,10 @@ Submodules:
   multiblock -- meshes of several blocks, their links and a driver with one
              handle per block
   restart -- restart files in the reference's layout
+  output  -- the reference's Tecplot files from downloaded fields
   fields  -- macroscopic fields and their relative L2 differences
   build   -- hipcc build of libgsis.so for gfx950 (and the flang host side)
 """
-from . import capi, cases, fields, multiblock, restart  # noqa: F401
+from . import capi, cases, fields, multiblock, output, restart  # noqa: F401
 
 __all__ = ["capi", "cases", "fields"]

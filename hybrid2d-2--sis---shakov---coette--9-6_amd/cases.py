@@ -560,7 +560,9 @@ def constants_from_control(c: dict) -> dict:
                 twall=c["Twall"] / c["T_inf"], dt=c["dt_global"], rcon=rcon,
                 if_viscous=int(c["If_viscous"]), scheme=int(c["Iflag_Scheme"]),
                 local_dt=int(c["Iflag_local_dt"]), cfl=c["CFL"], dtmax=c["dtmax"], dtmin=c["dtmin"],
-                kstep_show=int(c["Kstep_show"]))
+                kstep_show=int(c["Kstep_show"]),
+                # dimensional scales of the output files (src/IO.f90:532-550)
+                t_inf=c["T_inf"], nd_inf=c["ND_inf"], mole_mass=c["Mole_Mass"], lref=c["Lref"])
 
 
 def constants_from_kn(kn: float, omega: float = 0.81, pr: float = 2.0 / 3.0, gamma: float = 5.0 / 3.0,

@@ -293,6 +293,11 @@ class MultiBlockSolver:
                 break
         return done, hist
 
+    def write_output(self, directory, hybrid: bool = False):
+        """The reference's Tecplot files, one zone per block."""
+        from . import output
+        return output.write_output(directory, [(c, b.macro()) for c, b in zip(self.cases, self.blocks)], hybrid)
+
     # -- restart (src/Backup_write_read.f90) -------------------------------
     def save_restart(self, root, flag: int):
         """backup_Macro_write(flag): one record per block, then flag.dat."""

@@ -95,6 +95,12 @@ class GsisSolver:
     def set_counters(self, kstep: int, tt: float) -> None:
         capi.check(self.lib, self.lib.gsis_set_counters(self.h, int(kstep), float(tt)))
 
+    def write_output(self, directory, hybrid: bool = False):
+        """The reference's Tecplot files (output, src/IO.f90:307-553) from the
+        current fields."""
+        from . import output
+        return output.write_output(directory, [(self.case, self.macro())], hybrid)
+
     # -- inner faces between blocks (src/GKUA_boundary.f90:400-633) --------
     def inner_exchange(self, ksub: int, src: "GsisSolver", msub: int, orient: int) -> None:
         """Fill the ghost layers of subface ksub (1-based) from the cells next to

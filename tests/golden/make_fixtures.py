@@ -7,6 +7,7 @@ Fixtures are data only: the reference's input files for its two shipped
 cases, and known-answer values read from its recorded output files and from
 the numeric tables in its sources.
 """
+import hashlib
 import json
 import pathlib
 import re
@@ -79,6 +80,17 @@ def main(ref: pathlib.Path):
     nums = [float(t) for t in txt if re.match(r"^-?[0-9.]+(E[+-]?\d+)?$", t)]
     (HERE / "fort21.json").write_text(json.dumps({"KHS": nums[0], "Ma": nums[1], "Re": nums[2],
                                                   "Kn_inf": nums[3], "Rcon": nums[4]}))
+
+    # 6) SHA-256 and size of the output files that the shipped cylinder's
+    #    initial state determines (output, src/IO.f90:307-553); the tests
+    #    write them from the oracle's and the GPU's fields and compare bytes
+    names = ("flow2d.dat", "flow2d-GKUA.dat", "flow2d-S0.dat", "Tao_g_i.dat", "Tao_g_j.dat", "q_g_i.dat",
+             "q_g_j.dat", "flow2d_Dem.dat")
+    dig = {}
+    for n in names:
+        raw = (ref / "output" / n).read_bytes()
+        dig[n] = {"sha256": hashlib.sha256(raw).hexdigest(), "bytes": len(raw), "lines": raw.count(b"\n")}
+    (HERE / "output_digests.json").write_text(json.dumps(dig, indent=1))
     print("fixtures written to", HERE)
 
 
