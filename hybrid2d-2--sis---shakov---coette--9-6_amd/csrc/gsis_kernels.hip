@@ -1250,6 +1250,29 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
       if (lane == 0) st_agent(flag + sbp + (r - 2), epoch);
       flag_due = 0;
     }
+    // backward pass + update of row r-2 (src/GKUA_Residual.f90:64-109).  Its
+    // inputs are complete at the top of the step (Delta* of column i+1 was
+    // published in the previous step), and storing early gives the stores a
+    // whole step to complete before the next step's wait.  No wavefront reads
+    // F(i, r-2) after step s-2.
+    const int rf = r - 2;
+    if (rf >= 1 && rf <= nrow && !producer) {
+      T gr = T(0), hr = T(0);
+      if (i < ncol) {
+        gr = XG[pbuf][w + 1][lane].y;
+        hr = XH[pbuf][w + 1][lane].y;
+      }
+      const bool last = rf == nrow;  // Delta*(i, ny) = 0
+      const T gu = gd_o + (bx_o * gr + by_o * (last ? T(0) : gd_n));
+      const T hu = hd_o + (bx_o * hr + by_o * (last ? T(0) : hd_n));
+      T fg = g0 + gu, fh = h0 + hu;
+      if ((double)fg < kClamp) fg = T(kClamp);
+      if ((double)fh < kClamp) fh = T(kClamp);
+      if (act) {
+        stv(reinterpret_cast<T *>(Fb + oc - 2 * rb), voff, fg);
+        stv(reinterpret_cast<T *>(Fb + oc - 2 * rb + pb), voff, fh);
+      }
+    }
     asm volatile("" ::"s"(pf0), "s"(pf1), "s"(pf2), "s"(pf3));  // the previous step's touches
     if (r + 1 >= 1 && r + 1 <= nrow) {
       const int *pc = reinterpret_cast<const int *>(reinterpret_cast<const char *>(cpar) + ocp + ccstep);
@@ -1406,25 +1429,6 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
         stv(m + 64, loff, hd_new);
       }
     }
-    // backward pass + update of row r-2 (src/GKUA_Residual.f90:64-109)
-    const int rf = r - 2;
-    if (rf >= 1 && rf <= nrow && !producer) {
-      T gr = T(0), hr = T(0);
-      if (i < ncol) {
-        gr = XG[pbuf][w + 1][lane].y;
-        hr = XH[pbuf][w + 1][lane].y;
-      }
-      const bool last = rf == nrow;  // Delta*(i, ny) = 0
-      const T gu = gd_o + (bx_o * gr + by_o * (last ? T(0) : gd_n));
-      const T hu = hd_o + (bx_o * hr + by_o * (last ? T(0) : hd_n));
-      T fg = g0 + gu, fh = h0 + hu;
-      if ((double)fg < kClamp) fg = T(kClamp);
-      if ((double)fh < kClamp) fh = T(kClamp);
-      if (act) {
-        stv(reinterpret_cast<T *>(Fb + oc - 2 * rb), voff, fg);
-        stv(reinterpret_cast<T *>(Fb + oc - 2 * rb + pb), voff, fh);
-      }
-    }
     gd_o = gd_n; hd_o = hd_n; bx_o = bx_n; by_o = by_n;
     gd_n = gd_new; hd_n = hd_new; bx_n = bx_new; by_n = by_new;
     oc += rb;
@@ -1524,7 +1528,10 @@ __global__ void __launch_bounds__(256) k_res4(Dev d, const T *RG, const T *RH) {
 // of rows jv = l div nvi + R*k (R = 64 div nvi rows per pass when nvi <= 64),
 // or iv = l + 64*m for nvi > 64, so that vi/wi stay in registers and a pass
 // reads consecutive velocities.  Wave-level reduction only (no barrier).
-template <typename T>
+// ONE: 33 <= nvi <= 64, so a lane holds one column and each pass one whole
+// row; only the six row sums stay live through the loop, which leaves
+// registers for more loads in flight.
+template <typename T, bool ONE>
 __global__ void __launch_bounds__(256) k_moments(Dev d, T *FG, T *FH) {
   const int lane = threadIdx.x & 63;
   const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1544,10 +1551,49 @@ __global__ void __launch_bounds__(256) k_moments(Dev d, T *FG, T *FH) {
   double acc[10];
 #pragma unroll
   for (int k = 0; k < 10; k++) acc[k] = 0.0;
+  if constexpr (ONE) {
+    double A0 = 0.0, A1 = 0.0, A2 = 0.0, A3 = 0.0, A4 = 0.0, A5 = 0.0;
+    const int iv = min(lane, nvi - 1);
+    const bool on = lane < nvi;
+    const double vx = d.vi[iv], wx = d.wi[iv];
+#pragma unroll 8
+    for (int jv = 0; jv < nvj; jv++) {
+      const size_t ov = o + (size_t)jv * nvi + iv;
+      T g = FG[ov], h = FH[ov];
+      if (!interior && on) {
+        if ((double)g < kClamp) { g = T(kClamp); FG[ov] = g; }
+        if ((double)h < kClamp) { h = T(kClamp); FH[ov] = h; }
+      }
+      const double vy = d.vj[jv], wy = d.wj[jv];
+      const double w1 = wy * vy, w2 = w1 * vy, w3 = w2 * vy;
+      const double gd = (double)g, hd = (double)h;
+      A0 += wy * gd;
+      A1 += w1 * gd;
+      A2 += w2 * gd;
+      A5 += w3 * gd;
+      A3 += wy * hd;
+      A4 += w1 * hd;
+    }
+    const double vx2 = vx * vx;
+    const double e4 = A3 + vx2 * A0 + A2;
+    acc[0] = wx * A0;
+    acc[1] = wx * (vx * A0);
+    acc[2] = wx * A1;
+    acc[3] = wx * e4;
+    acc[4] = wx * (vx2 * A0);
+    acc[5] = wx * A2;
+    acc[6] = wx * (vx * A1);
+    acc[7] = wx * (vx * e4);
+    acc[8] = wx * (A4 + vx2 * A1 + A5);
+    acc[9] = wx * A3;
+    if (!on)  // lanes past the row add nothing
+#pragma unroll
+      for (int k = 0; k < 10; k++) acc[k] = 0.0;
+  }
   // R = 1 (nvi > 32): one row per pass, so the row index and its factors are
   // wavefront-uniform (scalar loads)
   const bool uni = R == 1;
-  for (int ch = 0; ch < nch; ch++) {
+  for (int ch = 0; ch < (ONE ? 0 : nch); ch++) {
     const int iv = nvi <= 64 ? lane % nvi : ch * 64 + lane;
     const int jr = uni ? 0 : lane / nvi;
     if (iv >= nvi || (uni ? (nvi <= 64 && lane >= nvi) : jr >= R)) continue;
@@ -2024,7 +2070,10 @@ bool launch_walk(const Dev &d, double *cpar, T *F, const T *slot, T *R, int stor
 
 template <typename T>
 void launch_moments(const Dev &d, T *FG, T *FH, hipStream_t s) {
-  hipLaunchKernelGGL(k_moments<T>, dim3((d.ncell + 3) / 4), dim3(256), 0, s, d, FG, FH);
+  if (d.nvi > 32 && d.nvi <= 64)
+    hipLaunchKernelGGL((k_moments<T, true>), dim3((d.ncell + 3) / 4), dim3(256), 0, s, d, FG, FH);
+  else
+    hipLaunchKernelGGL((k_moments<T, false>), dim3((d.ncell + 3) / 4), dim3(256), 0, s, d, FG, FH);
 }
 template <typename T>
 void launch_frd_in(const Dev &d, const double *frd, T *FG, T *FH, T *slot, hipStream_t s) {
