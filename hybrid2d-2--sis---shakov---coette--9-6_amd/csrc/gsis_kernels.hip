@@ -686,6 +686,18 @@ __device__ __forceinline__ T wflux(int f, int n, int fl, T a, T fm2, T fm1, T f0
   const bool wall = ((fl & 1) && f == 1) || ((fl & 2) && f == n);
   return wall ? a * sv : fx;
 }
+// wflux for the face at index 1 of its direction.  With n >= 5 (checked by
+// gsis_create) neither of wflux's n-1 / n rules can apply, and leaving their
+// comparisons out keeps loop-invariant masks of n out of scalar registers.
+template <int SL, typename T>
+__device__ __forceinline__ T wflux1(int fl, T a, T fm2, T fm1, T f0, T fp1, T sv, T hs) {
+  const T asp = posp(a), asn = negp(a);
+  const T d0 = f0 - fm1;
+  T sl, sr;
+  slopes<SL>(d0, fm1 - fm2, fp1 - f0, sl, sr);
+  const T fx = asp * (fm1 + hs * sl) + asn * (f0 - hs * sr);
+  return (fl & 1) ? a * sv : fx;
+}
 // interior face (no override) of the fused kernel's fast path
 template <int SL, typename T>
 __device__ __forceinline__ T iflux(T a, T fm2, T fm1, T f0, T fp1, T hs) {
@@ -1369,8 +1381,8 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
       // left face: boundary flux, the seam message or column i-1's value
       T gfl, gdl, hfl, hdl;
       if (xlo) {
-        gfl = wflux<SL>(1, nx, rfl, axl, cgxs, cgm1, g2, cgp1, cgxs, hs);
-        hfl = wflux<SL>(1, nx, rfl, axl, chxs, chm1, h2, chp1, chxs, hs);
+        gfl = wflux1<SL>(rfl, axl, cgxs, cgm1, g2, cgp1, cgxs, hs);
+        hfl = wflux1<SL>(rfl, axl, chxs, chm1, h2, chp1, chxs, hs);
         gdl = T(0);
         hdl = T(0);
       } else if (w == 0) {
@@ -1395,8 +1407,8 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
         hft = iflux<SL>(ayt, h1, h2, h3, h4, hs);
       }
       if (first) {
-        gfb = wflux<SL>(1, ny, cfl, ayb, g0, g1, g2, g3, cgys, hs);
-        hfb = wflux<SL>(1, ny, cfl, ayb, h0, h1, h2, h3, chys, hs);
+        gfb = wflux1<SL>(cfl, ayb, g0, g1, g2, g3, cgys, hs);
+        hfb = wflux1<SL>(cfl, ayb, h0, h1, h2, h3, chys, hs);
       } else {
         gfb = gfly;
         hfb = hfly;
