@@ -279,6 +279,19 @@ int gsis_create(const gsis_params *p, gsis_handle **out) {
       hipMemcpy(wj, p->wj, 8 * p->nvj, hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(GSIS_EHIP, "velocity grid upload failed"));
   d.vi = vi; d.vj = vj; d.wi = wi; d.wj = wj;
+  {  // row factors of the moments, in the order the moments kernel would form them
+    std::vector<double> rf(4 * (size_t)p->nvj);
+    for (int jv = 0; jv < p->nvj; jv++) {
+      const double wy = p->wj[jv], vy = p->vj[jv];
+      const double w1 = wy * vy, w2 = w1 * vy, w3 = w2 * vy;
+      rf[4 * jv] = wy; rf[4 * jv + 1] = w1; rf[4 * jv + 2] = w2; rf[4 * jv + 3] = w3;
+    }
+    double *rfd;
+    if ((rc = dalloc(h, &rfd, 8 * rf.size()))) return cleanup(rc);
+    if (hipMemcpy(rfd, rf.data(), 8 * rf.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(GSIS_EHIP, "velocity grid upload failed"));
+    d.rowf = rfd;
+  }
   // geometry (nx,ny) Fortran -> zero-padded (nx+2)x(ny+2)
   const size_t ng = (size_t)(nx + 2) * (ny + 2);
   const double *src[7] = {p->vol, p->si, p->sj, p->ni1, p->ni2, p->nj1, p->nj2};

@@ -33,6 +33,7 @@ struct Dev {
   int GI;             // geometry row length
   int L;              // slot length per face
   const double *vi, *vj, *wi, *wj;
+  const double *rowf;  // [nvj][4]: wy, wy*vy, wy*vy^2, wy*vy^3 of each velocity row (moments)
   const double *vol, *si, *sj, *ni1, *ni2, *nj1, *nj2;
   const double *xn, *yn;  // nodes, (nx+4) x (ny+4), index (j+1)*(nx+4)+(i+1)
   const double *x1, *y1;  // cell centres, padded cell grid

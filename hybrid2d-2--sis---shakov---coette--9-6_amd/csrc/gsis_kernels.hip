@@ -1540,11 +1540,30 @@ __global__ void __launch_bounds__(256) k_res4(Dev d, const T *RG, const T *RH) {
 // of rows jv = l div nvi + R*k (R = 64 div nvi rows per pass when nvi <= 64),
 // or iv = l + 64*m for nvi > 64, so that vi/wi stay in registers and a pass
 // reads consecutive velocities.  Wave-level reduction only (no barrier).
+// Raw buffer access (gfx9-family descriptor: word 3 = 0x00020000, offsets in
+// bytes, stride 0; a load past num_records returns 0).
+constexpr int kBufWord3 = 0x00020000;
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void buf_st(T v, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned int, v), r,
+                                          voff, soff, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+}
+
 // ONE: 33 <= nvi <= 64, so a lane holds one column and each pass one whole
 // row; only the six row sums stay live through the loop, which leaves
 // registers for more loads in flight.
 template <typename T, bool ONE>
-__global__ void __launch_bounds__(256) k_moments(Dev d, T *FG, T *FH) {
+__global__ void __launch_bounds__(256) k_moments(Dev d, T *__restrict__ FG, T *__restrict__ FH) {
   const int lane = threadIdx.x & 63;
   const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (c >= d.ncell) return;  // whole wavefronts: the reduction below stays wave-uniform
@@ -1568,16 +1587,26 @@ __global__ void __launch_bounds__(256) k_moments(Dev d, T *FG, T *FH) {
     const int iv = min(lane, nvi - 1);
     const bool on = lane < nvi;
     const double vx = d.vi[iv], wx = d.wi[iv];
+    // Buffer loads over the cell's velocities: the lane offset in a vector
+    // register, the row offset in a scalar one, so the unrolled rows need no
+    // 64-bit vector addresses (which took 32 registers as plain loads).
+    // Offsets stay below num_records = the cell's bytes.
+    const unsigned loff = (unsigned)iv * (unsigned)sizeof(T);
+    const int cell_bytes = nvi * nvj * (int)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rg = __builtin_amdgcn_make_buffer_rsrc(FG + o, (short)0, cell_bytes, kBufWord3);
+    const __amdgpu_buffer_rsrc_t rh = __builtin_amdgcn_make_buffer_rsrc(FH + o, (short)0, cell_bytes, kBufWord3);
 #pragma unroll 8
     for (int jv = 0; jv < nvj; jv++) {
-      const size_t ov = o + (size_t)jv * nvi + iv;
-      T g = FG[ov], h = FH[ov];
+      const int soff = jv * nvi * (int)sizeof(T);
+      T g = buf_ld<T>(rg, loff, soff), h = buf_ld<T>(rh, loff, soff);
       if (!interior && on) {
-        if ((double)g < kClamp) { g = T(kClamp); FG[ov] = g; }
-        if ((double)h < kClamp) { h = T(kClamp); FH[ov] = h; }
+        if ((double)g < kClamp) { g = T(kClamp); buf_st<T>(g, rg, loff, soff); }
+        if ((double)h < kClamp) { h = T(kClamp); buf_st<T>(h, rh, loff, soff); }
       }
-      const double vy = d.vj[jv], wy = d.wj[jv];
-      const double w1 = wy * vy, w2 = w1 * vy, w3 = w2 * vy;
+      // row factors from the table (scalar loads, scalar operands): formed
+      // here they would take vector registers for every unrolled row
+      const double wy = d.rowf[4 * jv], w1 = d.rowf[4 * jv + 1], w2 = d.rowf[4 * jv + 2],
+                   w3 = d.rowf[4 * jv + 3];
       const double gd = (double)g, hd = (double)h;
       A0 += wy * gd;
       A1 += w1 * gd;
