@@ -157,6 +157,12 @@ void orc_counters(const orc_t *o, int *kstep, double *tt, int *ns_iter) {
   if (ns_iter) *ns_iter = o->ns_iter;
 }
 
+/* the counters init_flow_read restores from flag.dat (src/Backup_write_read.f90:20-32) */
+void orc_set_counters(orc_t *o, int kstep, double tt) {
+  o->kstep = kstep;
+  o->tt = tt;
+}
+
 static double *fvel(const orc_t *o, double *base, int iv, int jv) { /* per-velocity plane */
   return base + FPLANE * ((size_t)(iv - 1) + (size_t)o->nvi * (jv - 1));
 }

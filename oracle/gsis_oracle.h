@@ -26,6 +26,7 @@ void orc_set_macro(orc_t *o, const double *s, const double *cf, const double *u,
 void orc_get_macro(const orc_t *o, double *s, double *s0, double *cf, double *u, double *taonsi_g,
                    double *taonsj_g, double *qi_g, double *qj_g, double *swall);
 void orc_counters(const orc_t *o, int *kstep, double *tt, int *ns_iter);
+void orc_set_counters(orc_t *o, int kstep, double tt);
 void orc_phy_boundary(orc_t *o);
 void orc_detfp(orc_t *o);
 int orc_convert_ns(orc_t *o);

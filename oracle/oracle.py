@@ -53,6 +53,8 @@ def load():
         lib.orc_set_macro.argtypes = [vp, dp, dp, dp, dp]
         lib.orc_get_macro.argtypes = [vp] + [dp] * 9
         lib.orc_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp, ctypes.POINTER(ctypes.c_int)]
+        lib.orc_set_counters.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        lib.orc_set_counters.restype = None
         lib.orc_phy_boundary.argtypes = [vp]
         lib.orc_detfp.argtypes = [vp]
         lib.orc_convert_ns.argtypes = [vp]
@@ -240,6 +242,9 @@ class OracleSolver:
         it = ctypes.c_int()
         self.lib.orc_counters(self.h, ctypes.byref(k), ctypes.byref(t), ctypes.byref(it))
         return k.value, t.value, it.value
+
+    def set_counters(self, kstep: int, tt: float) -> None:
+        self.lib.orc_set_counters(self.h, int(kstep), float(tt))
 
     def run(self, max_steps: int, tol: float = 1e-8, show: int = 50):
         hist = []

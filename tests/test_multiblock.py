@@ -344,3 +344,50 @@ def test_passive_faces_are_accepted_by_the_parameter_checks():
             assert rc == capi.GSIS_EARG and msg.startswith("BC 4:"), msg
         else:
             assert rc == capi.GSIS_EUNSUPPORTED and msg.startswith("BC 1:"), msg
+
+
+def _split_cavity_cases(n=8, nv=8, rot=True):
+    g = load_pkg()
+    whole = g.cases.cavity_case(n, nv, kn=0.1, lid=0.15, dt=0.5)
+    nx, ny = whole.nx, whole.ny
+    xn = np.asfortranarray(whole.mesh.x[2:nx + 2, 2:ny + 2])
+    yn = np.asfortranarray(whole.mesh.y[2:nx + 2, 2:ny + 2])
+    nodes, bcs = g.multiblock.split_nodes(xn, yn, whole.bcs, nx // 2 + 1, 0, rot)
+    return g.multiblock.block_cases(nodes, bcs, whole)
+
+
+def test_multiblock_driver_save_and_resume_on_oracle_blocks(tmp_path):
+    """MultiBlockSolver's restart path (backup_Mesco_write / backup_Mesco_read,
+    src/Backup_write_read.f90:56-209) on oracle blocks: one record per block in
+    block order, the step counter from flag.dat, and the inner-face exchange
+    after every block's distribution is back, so the inner ghost cells hold the
+    neighbour's saved cells (the exchange itself is pinned above)."""
+    g = load_pkg()
+    orc = load_oracle()
+    cases, links = _split_cavity_cases()
+
+    def make(c):
+        return orc.OracleSolver(c)
+
+    a = g.multiblock.MultiBlockSolver(cases, links, make=make)
+    a.initialize()
+    a.step(3)
+    a.save_restart(tmp_path, 2)
+    saved = a.frd()
+    res_a = a.residual()
+    a.close()
+    b = g.multiblock.MultiBlockSolver(cases, links, make=make)
+    b.initialize()
+    assert b.resume(tmp_path) == 2
+    assert b.counters()[0] == 3
+    exp = [f.copy() for f in saved]
+    for ln in links:
+        c1, c2 = cases[ln.block - 1], cases[ln.nblock - 1]
+        orc.inner_exchange_arrays(exp[ln.block - 1], np.zeros((11, c1.nx + 7, c1.ny + 7)), c1.bcs[ln.ksub - 1],
+                                  saved[ln.nblock - 1], np.zeros((11, c2.nx + 7, c2.ny + 7)), c2.bcs[ln.msub - 1],
+                                  ln.orient)
+    for f, e, f0 in zip(b.frd(), exp, saved):
+        np.testing.assert_array_equal(f[:, 1:-1, 1:-1], e[:, 1:-1, 1:-1])
+        assert not np.array_equal(e, f0)  # the exchange changed the inner ghost cells
+    assert np.all(res_a > 0)
+    b.close()
