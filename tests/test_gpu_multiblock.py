@@ -14,9 +14,10 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def _split_cavity(n=16, nv=16, kn=0.1, axis=0, rot=False, lid=0.15, dt=0.5):
+def _split_cavity(n=16, nv=16, kn=0.1, axis=0, rot=False, lid=0.15, dt=0.5, scheme=1):
     g = load_pkg()
     whole = g.cases.cavity_case(n, nv, kn=kn, lid=lid, dt=dt)
+    whole.const["scheme"] = scheme
     nx, ny = whole.nx, whole.ny
     xn = np.asfortranarray(whole.mesh.x[2:nx + 2, 2:ny + 2])
     yn = np.asfortranarray(whole.mesh.y[2:nx + 2, 2:ny + 2])
@@ -72,13 +73,15 @@ def test_inner_exchange_bitwise(prec, axis, rot):
     mbs.close()
 
 
-@pytest.mark.parametrize("axis,rot", [(0, False), (1, True)])
-def test_two_block_cavity_parity(axis, rot):
+@pytest.mark.parametrize("axis,rot,scheme", [(0, False, 1), (1, True, 1), (0, True, 3), (1, False, 0)],
+                         ids=["i-nnd", "j-turned-nnd", "i-turned-weno3", "j-up1"])
+def test_two_block_cavity_parity(axis, rot, scheme):
     """Both blocks step independently between exchanges, as between two
-    restarts of the reference; the GPU blocks follow the oracle blocks."""
+    restarts of the reference; the GPU blocks follow the oracle blocks, with
+    each flux scheme."""
     g = load_pkg()
-    cases, links = _split_cavity(n=16, nv=16, axis=axis, rot=rot)
-    print("two-block cavity axis", axis, "rot", rot, [c.name for c in cases], flush=True)
+    cases, links = _split_cavity(n=16, nv=16, axis=axis, rot=rot, scheme=scheme)
+    print("two-block cavity axis", axis, "rot", rot, "scheme", scheme, [c.name for c in cases], flush=True)
     gpu = g.multiblock.MultiBlockSolver(cases, links)
     cpu = _oracle_blocks(cases, links)
     for s in (gpu, cpu):
