@@ -155,7 +155,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the velocity-space sharding across GPUs is described in DESIGN.md, not built.", file=sys.stderr)
+              "sharding velocity space across GPUs is not built.", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe > 0:

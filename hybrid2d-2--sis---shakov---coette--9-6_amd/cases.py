@@ -1,7 +1,7 @@
 """Host-side producers of the GSIS kernel inputs.
 
-These mirror what the Fortran host computes before the first step (SURVEY
-row A17): the mesh geometry (Creat_Mesh, src/sub_init.f90:88-219, and the
+These mirror what the Fortran host computes before the first step: the mesh
+geometry (Creat_Mesh, src/sub_init.f90:88-219, and the
 ghost-node extrapolation of Update_coordinate_buffer_onemesh,
 src/sub_init.f90:719-823), the discrete velocity grid (GKUA_lisansudu,
 src/GKUA_lisansudu.f90:4-353), the constants of ``init`` (src/sub_init.f90:6-79)
