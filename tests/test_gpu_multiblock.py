@@ -163,3 +163,32 @@ def test_multiblock_backup_and_resume(tmp_path):
     _compare_blocks(b, cpu, "resumed +5")
     b.close()
     cpu.close()
+
+
+def test_inner_exchange_argument_errors():
+    """gsis_inner_exchange refuses, with GSIS_EARG and before any device work,
+    a receiving subface that is not an inner face, subface numbers out of
+    range, faces of different cell counts and blocks of different precisions."""
+    g = load_pkg()
+    capi = g.capi
+    cases, links = _split_cavity(n=12, nv=8, axis=0, rot=False)
+    ln = links[0]
+    a = g.multiblock.MultiBlockSolver(cases, links)
+    b32 = solver_mod().GsisSolver(cases[1], precision=1)
+    dst, src = a.blocks[ln.block - 1], a.blocks[ln.nblock - 1]
+    wall = [k for k, c in enumerate(cases[ln.block - 1].bcs, start=1) if c.code < 1][0]
+    other_face = [k for k, c in enumerate(cases[ln.nblock - 1].bcs, start=1)
+                  if (c.iend - c.ist + c.jend - c.jst) != (cases[ln.block - 1].bcs[ln.ksub - 1].iend
+                                                          - cases[ln.block - 1].bcs[ln.ksub - 1].ist
+                                                          + cases[ln.block - 1].bcs[ln.ksub - 1].jend
+                                                          - cases[ln.block - 1].bcs[ln.ksub - 1].jst)]
+    bad = [(dst, wall, src, ln.msub), (dst, 0, src, ln.msub), (dst, ln.ksub, src, 99), (dst, ln.ksub, b32, ln.msub)]
+    if other_face:
+        bad.append((dst, ln.ksub, src, other_face[0]))
+    for d, k, s_, m in bad:
+        with pytest.raises(capi.GsisError) as e:
+            d.inner_exchange(k, s_, m, ln.orient)
+        assert e.value.code == capi.GSIS_EARG, str(e.value)
+    a.exchange()  # the handles are still usable
+    b32.close()
+    a.close()
