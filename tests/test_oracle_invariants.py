@@ -486,3 +486,66 @@ def test_uniform_flow_is_preserved_by_the_m1_step():
     assert np.max(np.abs(f1[cf_] - f0[cf_])) < 1e-12 * np.max(f0[cf_])
     # the flow is really moving, and in both directions
     assert m1["u"][1][cu[1:]].min() > 0.5 and m1["u"][2][cu[1:]].min() > 0.3
+
+
+def test_wall_moments_exact_on_linear_fields():
+    """DETFPwall (src/GKUA_DETFP_WALL.f90:82-83) extrapolates the moments of the
+    first two cells to each wall node and sets the ghost cell so that it and
+    the first cell average to it.  With a density linear in x and y (a
+    Maxwellian scaled cell by cell) on a uniform mesh, every moment is either
+    extensive (the density times its unit value) or intensive (its unit value),
+    so affine in space: the wall value is the moment at the wall face and the
+    ghost value the moment at the ghost cell centre, exactly."""
+    g = load_pkg()
+    case = g.cases.cavity_case(8, 12, kn=0.1, lid=0.0, dt=0.5)
+    o = load_oracle().OracleSolver(case)
+    o.initialize()
+    nx, ny = case.nx, case.ny
+    h = 1.0 / (nx - 1)
+    a, b = 0.3, -0.2
+
+    def dens(xc, yc):
+        return 1.0 + a * xc + b * yc
+
+    unit = o.macro()["s"][:10, 4, 4].copy()  # the uniform state's moments
+    F = o.frd()
+    for i in range(-1, nx + 2):
+        for j in range(-1, ny + 2):
+            F[:, i + 3, j + 3] *= dens((i - 0.5) * h, (j - 0.5) * h)  # cell (i, j) centre
+    o.upload(frd=F)
+    o.refresh_moments()
+    m = o.macro()
+    s, sw = m["s"], m["swall"]
+    big = np.abs(unit) > 1e-12
+    d11 = dens(0.5 * h, 0.5 * h)
+    ext = big & (np.abs(s[:10, 4, 4] - d11 * unit) < 1e-13)
+    inten = big & (np.abs(s[:10, 4, 4] - unit) < 1e-13)
+    assert ext[0] and inten[3] and np.all(ext | inten | ~big), (ext, inten)  # density, temperature
+
+    def expect(d):
+        return np.where(ext, d * unit, unit)
+
+    off = 0
+    checked = 0
+    for bc in case.bcs:
+        nk = (bc.jend - bc.jst + 1) if bc.face in (1, 3) else (bc.iend - bc.ist + 1)
+        w = sw[off:off + 10 * nk].reshape((10, nk), order="F")
+        for k in range(nk - 1):  # the last node of a face is skipped (:65-67)
+            if bc.face in (1, 3):
+                j = bc.jst + k
+                xw, yc = (0.0 if bc.face == 1 else 1.0), (j - 0.5) * h
+                sgn = -1.0 if bc.face == 1 else 1.0
+                face, ghost = dens(xw, yc), dens(xw + 0.5 * sgn * h, yc)
+                gi, gj = (0, j) if bc.face == 1 else (nx, j)
+            else:
+                i = bc.ist + k
+                xc, yw = (i - 0.5) * h, (0.0 if bc.face == 2 else 1.0)
+                sgn = -1.0 if bc.face == 2 else 1.0
+                face, ghost = dens(xc, yw), dens(xc, yw + 0.5 * sgn * h)
+                gi, gj = (i, 0) if bc.face == 2 else (i, ny)
+            np.testing.assert_allclose(w[big, k], expect(face)[big], rtol=0, atol=1e-13, err_msg=str((bc.face, k)))
+            np.testing.assert_allclose(s[:10, gi + 3, gj + 3][big], expect(ghost)[big], rtol=0, atol=1e-13,
+                                       err_msg=str((bc.face, k)))
+            checked += 1
+        off += 10 * nk
+    assert checked == 4 * 8
