@@ -1561,12 +1561,20 @@ __device__ __forceinline__ void buf_st(T v, __amdgpu_buffer_rsrc_t r, unsigned v
 
 // ONE: 33 <= nvi <= 64, so a lane holds one column and each pass one whole
 // row; only the six row sums stay live through the loop, which leaves
-// registers for more loads in flight.
+// registers for more loads in flight.  Two wavefronts share a cell, each over
+// half of its rows, and combine through LDS: twice the wavefronts shortens the
+// last partial round of the grid.
 template <typename T, bool ONE>
 __global__ void __launch_bounds__(256) k_moments(Dev d, T *__restrict__ FG, T *__restrict__ FH) {
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (c >= d.ncell) return;  // whole wavefronts: the reduction below stays wave-uniform
+  // the wavefront index through readfirstlane, so that the row range below is
+  // known to be uniform (scalar row offsets and factors)
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int c = ONE ? blockIdx.x * 2 + (wv >> 1) : blockIdx.x * 4 + wv;
+  const int half = ONE ? (wv & 1) : 0;
+  // whole wavefronts: the reduction below stays wave-uniform; the ONE path
+  // returns only after its barrier
+  if (!ONE && c >= d.ncell) return;
+  const bool valid = c < d.ncell;
   const int i = c % d.NI - 1, j = c / d.NI - 1;
   const bool interior = (i >= 1 && i <= d.nx - 1 && j >= 1 && j <= d.ny - 1);
   const bool ring = (i >= 0 && i <= d.nx && j >= 0 && j <= d.ny);
@@ -1595,8 +1603,10 @@ __global__ void __launch_bounds__(256) k_moments(Dev d, T *__restrict__ FG, T *_
     const int cell_bytes = nvi * nvj * (int)sizeof(T);
     const __amdgpu_buffer_rsrc_t rg = __builtin_amdgcn_make_buffer_rsrc(FG + o, (short)0, cell_bytes, kBufWord3);
     const __amdgpu_buffer_rsrc_t rh = __builtin_amdgcn_make_buffer_rsrc(FH + o, (short)0, cell_bytes, kBufWord3);
+    const int hn = (nvj + 1) >> 1;
+    const int j0 = valid ? half * hn : 0, j1 = valid ? min(nvj, half * hn + hn) : 0;
 #pragma unroll 8
-    for (int jv = 0; jv < nvj; jv++) {
+    for (int jv = j0; jv < j1; jv++) {
       const int soff = jv * nvi * (int)sizeof(T);
       T g = buf_ld<T>(rg, loff, soff), h = buf_ld<T>(rh, loff, soff);
       if (!interior && on) {
@@ -1630,6 +1640,17 @@ __global__ void __launch_bounds__(256) k_moments(Dev d, T *__restrict__ FG, T *_
     if (!on)  // lanes past the row add nothing
 #pragma unroll
       for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    __shared__ double part[2][10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] = wave_sum_last(acc[k]);
+    if (half == 1 && lane == 63)
+#pragma unroll
+      for (int k = 0; k < 10; k++) part[wv >> 1][k] = acc[k];
+    __syncthreads();
+    if (!valid || half == 1) return;
+    if (lane == 63)
+#pragma unroll
+      for (int k = 0; k < 10; k++) acc[k] += part[wv >> 1][k];
   }
   // R = 1 (nvi > 32): one row per pass, so the row index and its factors are
   // wavefront-uniform (scalar loads)
@@ -1683,8 +1704,9 @@ __global__ void __launch_bounds__(256) k_moments(Dev d, T *__restrict__ FG, T *_
     if (lane < 11) d.S[lane * d.ncell + c] = 0.0;
     return;
   }
+  if constexpr (!ONE)
 #pragma unroll
-  for (int k = 0; k < 10; k++) acc[k] = wave_sum_last(acc[k]);
+    for (int k = 0; k < 10; k++) acc[k] = wave_sum_last(acc[k]);
   if (lane == 63) {
     double ron = acc[0];
     if (ron <= 0.) ron = 1.0;  // R00
@@ -2112,7 +2134,7 @@ bool launch_walk(const Dev &d, double *cpar, T *F, const T *slot, T *R, int stor
 template <typename T>
 void launch_moments(const Dev &d, T *FG, T *FH, hipStream_t s) {
   if (d.nvi > 32 && d.nvi <= 64)
-    hipLaunchKernelGGL((k_moments<T, true>), dim3((d.ncell + 3) / 4), dim3(256), 0, s, d, FG, FH);
+    hipLaunchKernelGGL((k_moments<T, true>), dim3((d.ncell + 1) / 2), dim3(256), 0, s, d, FG, FH);
   else
     hipLaunchKernelGGL((k_moments<T, false>), dim3((d.ncell + 3) / 4), dim3(256), 0, s, d, FG, FH);
 }
