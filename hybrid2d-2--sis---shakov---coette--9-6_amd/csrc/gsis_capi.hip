@@ -237,6 +237,55 @@ int check_err_flag(gsis_handle *h) {
   return 0;
 }
 
+// FRD between the host's Fortran array and the device layout, a chunk of
+// consecutive velocities at a time through a device stage (each velocity's
+// 2(nx+7)(ny+7) doubles are contiguous in the Fortran array).  With stage ==
+// nullptr a stage of at most kStageBytes (at least one velocity) is allocated
+// for the call.
+constexpr size_t kStageBytes = (size_t)64 << 20;
+
+int frd_transfer(gsis_handle *h, double *frd, bool up, double *stage, size_t stage_bytes) {
+  const Dev &d = h->d;
+  const size_t blk = (size_t)2 * (d.nx + 7) * (d.ny + 7);
+  bool own = false;
+  if (!stage) {
+    stage_bytes = std::max(blk * 8, std::min(blk * 8 * (size_t)d.nv, kStageBytes));
+    HIPCHK(hipMalloc(&stage, stage_bytes));
+    own = true;
+  }
+  const int nvc = (int)std::min((size_t)d.nv, stage_bytes / (blk * 8));
+  hipError_t e = hipSuccess;
+  for (int v0 = 0; v0 < d.nv && e == hipSuccess; v0 += nvc) {
+    const int n = std::min(nvc, d.nv - v0);
+    double *host = frd + blk * (size_t)v0;
+    if (up) {
+      e = hipMemcpyAsync(stage, host, blk * n * 8, hipMemcpyHostToDevice, h->s);
+      if (e != hipSuccess) break;
+      if (h->prec == GSIS_PREC_FP32)
+        gsis::launch_frd_in_chunk<float>(d, stage, v0, n, (float *)h->FG, (float *)h->FH, (float *)h->slot, h->s);
+      else
+        gsis::launch_frd_in_chunk<double>(d, stage, v0, n, (double *)h->FG, (double *)h->FH, (double *)h->slot,
+                                          h->s);
+    } else {
+      if (h->prec == GSIS_PREC_FP32)
+        gsis::launch_frd_out_chunk<float>(d, stage, v0, n, (const float *)h->FG, (const float *)h->FH,
+                                          (const float *)h->slot, h->s);
+      else
+        gsis::launch_frd_out_chunk<double>(d, stage, v0, n, (const double *)h->FG, (const double *)h->FH,
+                                           (const double *)h->slot, h->s);
+      e = hipGetLastError();
+      if (e != hipSuccess) break;
+      e = hipMemcpyAsync(host, stage, blk * n * 8, hipMemcpyDeviceToHost, h->s);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+  if (own) (void)hipFree(stage);
+  if (e != hipSuccess)
+    return fail(GSIS_EHIP, std::string(up ? "upload_frd: " : "download_frd: ") + hipGetErrorString(e));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -464,38 +513,12 @@ int gsis_destroy(gsis_handle *h) {
 
 int gsis_upload_frd(gsis_handle *h, const double *frd) {
   if (!h || !frd) return fail(GSIS_EARG, "NULL argument");
-  const Dev &d = h->d;
-  const size_t n = (size_t)2 * (d.nx + 7) * (d.ny + 7) * d.nv;
-  double *tmp = nullptr;
-  HIPCHK(hipMalloc(&tmp, n * 8));
-  hipError_t e = hipMemcpyAsync(tmp, frd, n * 8, hipMemcpyHostToDevice, h->s);
-  if (e == hipSuccess) {
-    if (h->prec == GSIS_PREC_FP32) gsis::launch_frd_in<float>(d, tmp, (float *)h->FG, (float *)h->FH, (float *)h->slot, h->s);
-    else gsis::launch_frd_in<double>(d, tmp, (double *)h->FG, (double *)h->FH, (double *)h->slot, h->s);
-    e = hipStreamSynchronize(h->s);
-  }
-  (void)hipFree(tmp);
-  if (e != hipSuccess) return fail(GSIS_EHIP, std::string("upload_frd: ") + hipGetErrorString(e));
-  HIPCHK(hipGetLastError());
-  return 0;
+  return frd_transfer(h, const_cast<double *>(frd), true, nullptr, 0);
 }
 
 int gsis_download_frd(gsis_handle *h, double *frd) {
   if (!h || !frd) return fail(GSIS_EARG, "NULL argument");
-  const Dev &d = h->d;
-  const size_t n = (size_t)2 * (d.nx + 7) * (d.ny + 7) * d.nv;
-  double *tmp = nullptr;
-  HIPCHK(hipMalloc(&tmp, n * 8));
-  hipError_t e = hipMemsetAsync(tmp, 0, n * 8, h->s);
-  if (e == hipSuccess) {
-    if (h->prec == GSIS_PREC_FP32) gsis::launch_frd_out<float>(d, tmp, (const float *)h->FG, (const float *)h->FH, (const float *)h->slot, h->s);
-    else gsis::launch_frd_out<double>(d, tmp, (const double *)h->FG, (const double *)h->FH, (const double *)h->slot, h->s);
-    e = hipMemcpyAsync(frd, tmp, n * 8, hipMemcpyDeviceToHost, h->s);
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(h->s);
-  (void)hipFree(tmp);
-  if (e != hipSuccess) return fail(GSIS_EHIP, std::string("download_frd: ") + hipGetErrorString(e));
-  return 0;
+  return frd_transfer(h, frd, false, nullptr, 0);
 }
 
 int gsis_upload_macro(gsis_handle *h, const double *s, const double *cf, const double *u, const double *u0) {

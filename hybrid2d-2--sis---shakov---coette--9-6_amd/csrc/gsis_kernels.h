@@ -50,8 +50,13 @@ size_t strip_rows(const Dev &d, int nw);
 // the seam update; false when the mesh or the buffers do not fit.
 template <typename T> bool launch_strip(const Dev &d, double *cpar, T *F, const T *slot, T *R, int store_rhs, int nw, StripBufs &b, hipStream_t s);
 template <typename T> void launch_res4(const Dev &d, const T *RG, const T *RH, hipStream_t s);
-template <typename T> void launch_frd_in(const Dev &d, const double *frd, T *FG, T *FH, T *slot, hipStream_t s);
-template <typename T> void launch_frd_out(const Dev &d, double *frd, const T *FG, const T *FH, const T *slot, hipStream_t s);
+// FRD (Fortran layout) to and from the device layout for velocities v0..v0+nvc-1 through a stage holding that
+// contiguous slice of the Fortran FRD array (2(nx+7)(ny+7) doubles per velocity)
+template <typename T>
+void launch_frd_out_chunk(const Dev &d, double *stage, int v0, int nvc, const T *FG, const T *FH, const T *slot,
+                          hipStream_t s);
+template <typename T>
+void launch_frd_in_chunk(const Dev &d, const double *stage, int v0, int nvc, T *FG, T *FH, T *slot, hipStream_t s);
 void launch_save_s0(const Dev &d, hipStream_t s);
 void launch_det_wall(const Dev &d, const int *list, int n, double *swall, hipStream_t s);
 void launch_convert(const Dev &d, hipStream_t s);

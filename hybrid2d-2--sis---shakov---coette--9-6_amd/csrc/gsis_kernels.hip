@@ -1907,64 +1907,53 @@ __global__ void k_residual(Dev d, double *out) {
 // Layout transposition between the reference's FRD(2,-3:nx+3,-3:ny+3,nvi,nvj)
 // and the device's f[cell][velocity] planes + wall slots.
 // ------------------------------------------------------------------------
-__device__ __forceinline__ size_t frd_idx(const Dev &d, int ird, int i, int j, int v) {
-  return (size_t)ird + 2 * ((size_t)(i + 3) + (size_t)(d.nx + 7) * ((size_t)(j + 3) + (size_t)(d.ny + 7) * v));
+
+// A chunk of consecutive velocities v0..v0+nvc-1 at a time.  The Fortran
+// array holds each velocity's values as one contiguous block of
+// 2(nx+7)(ny+7) doubles, so a chunk is a contiguous slice of it; the stage
+// holds that slice, element ird + 2((i+3) + (nx+7)((j+3) + (ny+7)(v-v0))).
+// One thread per stage element.  Every stage element is written (zero where
+// the device keeps no value), so the stage needs no clearing.
+template <typename T>
+__global__ void k_frd_out_chunk(Dev d, double *stage, int v0, int nvc, const T *FG, const T *FH, const T *slot) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nxp = d.nx + 7, nyp = d.ny + 7;
+  if (t >= 2 * nxp * nyp * (size_t)nvc) return;
+  const int ird = (int)(t & 1);
+  const size_t q = t >> 1;
+  const int i = (int)(q % nxp) - 3, j = (int)((q / nxp) % nyp) - 3;
+  const int v = v0 + (int)(q / (nxp * nyp));
+  double val = 0.0;
+  if (i >= -1 && i <= d.nx + 1 && j >= -1 && j <= d.ny + 1) {
+    const size_t f = (size_t)cidx(d, i, j) * d.nv + v;
+    val = (double)(ird == 0 ? FG[f] : FH[f]);
+  } else if ((i == -3 || i == d.nx + 3) && j >= -1 && j <= d.ny + 1) {
+    val = (double)slot[slot_idx(d, i < 0 ? 1 : 3, j + 1, ird, v)];
+  } else if ((j == -3 || j == d.ny + 3) && i >= -1 && i <= d.nx + 1) {
+    val = (double)slot[slot_idx(d, j < 0 ? 2 : 4, i + 1, ird, v)];
+  }
+  stage[t] = val;
 }
 
 template <typename T>
-__global__ void k_frd_in(Dev d, const double *frd, T *FG, T *FH, T *slot) {
+__global__ void k_frd_in_chunk(Dev d, const double *stage, int v0, int nvc, T *FG, T *FH, T *slot) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t ncv = (size_t)d.ncell * d.nv;
-  if (t < ncv) {
-    const int v = (int)(t % d.nv);
-    const int c = (int)(t / d.nv);
-    const int i = c % d.NI - 1, j = c / d.NI - 1;
-    FG[t] = T(frd[frd_idx(d, 0, i, j, v)]);
-    FH[t] = T(frd[frd_idx(d, 1, i, j, v)]);
-    return;
+  const size_t nxp = d.nx + 7, nyp = d.ny + 7;
+  if (t >= 2 * nxp * nyp * (size_t)nvc) return;
+  const int ird = (int)(t & 1);
+  const size_t q = t >> 1;
+  const int i = (int)(q % nxp) - 3, j = (int)((q / nxp) % nyp) - 3;
+  const int v = v0 + (int)(q / (nxp * nyp));
+  const T val = T(stage[t]);
+  if (i >= -1 && i <= d.nx + 1 && j >= -1 && j <= d.ny + 1) {
+    const size_t f = (size_t)cidx(d, i, j) * d.nv + v;
+    if (ird == 0) FG[f] = val;
+    else FH[f] = val;
+  } else if ((i == -3 || i == d.nx + 3) && j >= -1 && j <= d.ny + 1) {
+    slot[slot_idx(d, i < 0 ? 1 : 3, j + 1, ird, v)] = val;
+  } else if ((j == -3 || j == d.ny + 3) && i >= -1 && i <= d.nx + 1) {
+    slot[slot_idx(d, j < 0 ? 2 : 4, i + 1, ird, v)] = val;
   }
-  const size_t u = t - ncv;
-  const size_t nsl = (size_t)4 * d.L * d.nv;
-  if (u >= nsl) return;
-  const int v = (int)(u % d.nv);
-  const int pos = (int)((u / d.nv) % d.L);
-  const int face = (int)(u / ((size_t)d.nv * d.L)) + 1;
-  int i, j;
-  if (face == 1) { i = -3; j = pos - 1; }
-  else if (face == 3) { i = d.nx + 3; j = pos - 1; }
-  else if (face == 2) { i = pos - 1; j = -3; }
-  else { i = pos - 1; j = d.ny + 3; }
-  const bool ok = (face == 1 || face == 3) ? (j <= d.ny + 1) : (i <= d.nx + 1);
-  for (int ird = 0; ird < 2; ird++)
-    slot[slot_idx(d, face, pos, ird, v)] = ok ? T(frd[frd_idx(d, ird, i, j, v)]) : T(0);
-}
-
-template <typename T>
-__global__ void k_frd_out(Dev d, double *frd, const T *FG, const T *FH, const T *slot) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t ncv = (size_t)d.ncell * d.nv;
-  if (t < ncv) {
-    const int v = (int)(t % d.nv);
-    const int c = (int)(t / d.nv);
-    const int i = c % d.NI - 1, j = c / d.NI - 1;
-    frd[frd_idx(d, 0, i, j, v)] = (double)FG[t];
-    frd[frd_idx(d, 1, i, j, v)] = (double)FH[t];
-    return;
-  }
-  const size_t u = t - ncv;
-  const size_t nsl = (size_t)4 * d.L * d.nv;
-  if (u >= nsl) return;
-  const int v = (int)(u % d.nv);
-  const int pos = (int)((u / d.nv) % d.L);
-  const int face = (int)(u / ((size_t)d.nv * d.L)) + 1;
-  int i, j;
-  if (face == 1) { i = -3; j = pos - 1; }
-  else if (face == 3) { i = d.nx + 3; j = pos - 1; }
-  else if (face == 2) { i = pos - 1; j = -3; }
-  else { i = pos - 1; j = d.ny + 3; }
-  const bool ok = (face == 1 || face == 3) ? (j <= d.ny + 1) : (i <= d.nx + 1);
-  if (ok)
-    for (int ird = 0; ird < 2; ird++) frd[frd_idx(d, ird, i, j, v)] = (double)slot[slot_idx(d, face, pos, ird, v)];
 }
 
 // ------------------------------------------------------------------------
@@ -2139,14 +2128,17 @@ void launch_moments(const Dev &d, T *FG, T *FH, hipStream_t s) {
     hipLaunchKernelGGL((k_moments<T, false>), dim3((d.ncell + 3) / 4), dim3(256), 0, s, d, FG, FH);
 }
 template <typename T>
-void launch_frd_in(const Dev &d, const double *frd, T *FG, T *FH, T *slot, hipStream_t s) {
-  const size_t n = (size_t)d.ncell * d.nv + (size_t)4 * d.L * d.nv;
-  hipLaunchKernelGGL(k_frd_in<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, frd, FG, FH, slot);
+void launch_frd_out_chunk(const Dev &d, double *stage, int v0, int nvc, const T *FG, const T *FH, const T *slot,
+                          hipStream_t s) {
+  const size_t n = (size_t)2 * (d.nx + 7) * (d.ny + 7) * nvc;
+  hipLaunchKernelGGL(k_frd_out_chunk<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, stage, v0, nvc, FG,
+                     FH, slot);
 }
 template <typename T>
-void launch_frd_out(const Dev &d, double *frd, const T *FG, const T *FH, const T *slot, hipStream_t s) {
-  const size_t n = (size_t)d.ncell * d.nv + (size_t)4 * d.L * d.nv;
-  hipLaunchKernelGGL(k_frd_out<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, frd, FG, FH, slot);
+void launch_frd_in_chunk(const Dev &d, const double *stage, int v0, int nvc, T *FG, T *FH, T *slot, hipStream_t s) {
+  const size_t n = (size_t)2 * (d.nx + 7) * (d.ny + 7) * nvc;
+  hipLaunchKernelGGL(k_frd_in_chunk<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, stage, v0, nvc, FG,
+                     FH, slot);
 }
 
 void launch_save_s0(const Dev &d, hipStream_t s) {
@@ -2184,8 +2176,9 @@ void launch_residual(const Dev &d, double *out, hipStream_t s) {
   template bool launch_walk<T>(const Dev &, double *, T *, const T *, T *, int, int, hipStream_t);      \
   template bool launch_strip<T>(const Dev &, double *, T *, const T *, T *, int, int, StripBufs &, hipStream_t); \
   template void launch_res4<T>(const Dev &, const T *, const T *, hipStream_t);                  \
-  template void launch_frd_in<T>(const Dev &, const double *, T *, T *, T *, hipStream_t);        \
-  template void launch_frd_out<T>(const Dev &, double *, const T *, const T *, const T *, hipStream_t);
+  template void launch_frd_out_chunk<T>(const Dev &, double *, int, int, const T *, const T *, const T *,       \
+                                        hipStream_t);                                                           \
+  template void launch_frd_in_chunk<T>(const Dev &, const double *, int, int, T *, T *, T *, hipStream_t);
 INST(double)
 INST(float)
 #undef INST
