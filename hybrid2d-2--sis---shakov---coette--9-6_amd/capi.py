@@ -52,6 +52,7 @@ EXPORTED = (
     "gsis_residual",
     "gsis_download_macro",
     "gsis_download_frd",
+    "gsis_download_frd_noalloc",
     "gsis_counters",
     "gsis_set_counters",
     "gsis_inner_exchange",
@@ -155,6 +156,7 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.gsis_residual.argtypes = [h, c_dp]
     lib.gsis_download_macro.argtypes = [h] + [c_dp] * 9
     lib.gsis_download_frd.argtypes = [h, c_dp]
+    lib.gsis_download_frd_noalloc.argtypes = [h, c_dp]
     lib.gsis_counters.argtypes = [h, ctypes.POINTER(ctypes.c_int32), c_dp, ctypes.POINTER(ctypes.c_int32)]
     lib.gsis_set_counters.argtypes = [h, ctypes.c_int32, ctypes.c_double]
     lib.gsis_inner_exchange.argtypes = [h, ctypes.c_int32, h, ctypes.c_int32, ctypes.c_int32]

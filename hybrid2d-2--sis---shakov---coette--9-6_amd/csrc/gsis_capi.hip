@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -286,6 +287,33 @@ int frd_transfer(gsis_handle *h, double *frd, bool up, double *stage, size_t sta
   return 0;
 }
 
+// Stage of gsis_download_frd_noalloc: a file-scope device array, one per
+// GPU, held until the process ends.  Host code never uses its name as an
+// address (that is a host-side stand-in); scratch_on() looks up each device's
+// copy once with hipGetSymbolAddress and keeps the pointer.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on(int dev, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices) return fail(GSIS_EARG, "device ordinal out of range for the scratch table");
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return fail(GSIS_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -519,6 +547,19 @@ int gsis_upload_frd(gsis_handle *h, const double *frd) {
 int gsis_download_frd(gsis_handle *h, double *frd) {
   if (!h || !frd) return fail(GSIS_EARG, "NULL argument");
   return frd_transfer(h, frd, false, nullptr, 0);
+}
+
+int gsis_download_frd_noalloc(gsis_handle *h, double *frd) {
+  if (!h || !frd) return fail(GSIS_EARG, "NULL argument");
+  const size_t blk_bytes = (size_t)2 * (h->d.nx + 7) * (h->d.ny + 7) * sizeof(double);
+  if (blk_bytes > kScratchBytes)
+    return fail(GSIS_EARG, "download_frd_noalloc: one velocity of FRD takes " + std::to_string(blk_bytes) +
+                               " bytes, more than the " + std::to_string(kScratchBytes) +
+                               "-byte device stage; use gsis_download_frd");
+  double *stage = nullptr;
+  const int rc = scratch_on(h->dev, &stage);
+  if (rc) return rc;
+  return frd_transfer(h, frd, false, stage, kScratchBytes);
 }
 
 int gsis_upload_macro(gsis_handle *h, const double *s, const double *cf, const double *u, const double *u0) {

@@ -88,6 +88,11 @@ module gsis_capi
       type(c_ptr), value :: h
       real(c_double), intent(out) :: frd(*)
     end function
+    integer(c_int) function gsis_download_frd_noalloc(h, frd) bind(C, name="gsis_download_frd_noalloc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: frd(*)
+    end function
     integer(c_int) function gsis_counters(h, kstep, tt, ns_iter) bind(C, name="gsis_counters")
       import :: c_int, c_ptr, c_int32_t, c_double
       type(c_ptr), value :: h

@@ -166,6 +166,16 @@ int gsis_download_macro(gsis_handle *h, double *s, double *s0, double *cf, doubl
  * (src/Backup_write_read.f90:150-209). */
 int gsis_download_frd(gsis_handle *h, double *frd);
 
+/* gsis_download_frd without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The transpose goes through a 32 MiB device stage that the library holds in
+ * a file-scope array on each GPU until the process ends, a chunk of whole
+ * velocities at a time.  Returns GSIS_EARG when one velocity of FRD,
+ * 2(nx+7)(ny+7) doubles, does not fit the stage.  Calls on handles of the same
+ * GPU share that stage: calls that overlap in time on one GPU race. */
+int gsis_download_frd_noalloc(gsis_handle *h, double *frd);
+
 /* Step counter and accumulated time (Mesh(1)%Kstep, Mesh(1)%tt,
  * src/Hybrid_Sub.f90:156-157) and, in mode M1, the last inner iteration count
  * (MP%iter). */
