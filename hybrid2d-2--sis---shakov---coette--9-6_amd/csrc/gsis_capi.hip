@@ -213,7 +213,7 @@ int one_step(gsis_handle *h) {
   gsis::launch_convert(h->d, h->s);
   if (h->p.if_viscous) gsis::launch_hot(h->d, h->s);
   if (h->p.mode == GSIS_MODE_M1) {
-    int rc = gsis::synthetic_step(h->d, h->syn, h->s, &h->ns_iter);
+    int rc = gsis::synthetic_step(h->d, h->syn, h->s);
     if (rc) return fail(rc, gsis::synthetic_error());
   }
   HIPCHK(hipGetLastError());

@@ -61,6 +61,17 @@ __host__ __device__ inline size_t slot_idx(const Dev &d, int face, int pos, int 
   return ((((size_t)(face - 1) * d.L + pos) * 2 + ird) * (size_t)d.nv) + v;
 }
 
+// Loads and stores coherent across the whole device (all XCDs), for values
+// one workgroup hands to another during a launch.
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // One boundary subface prepared for the device.
 struct BcDev {
   int face, ist, iend, jst, jend, code;

@@ -1090,14 +1090,6 @@ __global__ void __launch_bounds__(1024) k_walk(Dev d, const double *cpar, int VT
 // its old values, which are the ones its neighbours read.
 // ------------------------------------------------------------------------
 constexpr int kSpinMax = 1 << 20;  // about a second of polling
-template <typename T>
-__device__ __forceinline__ T ld_agent(const T *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void st_agent(T *p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // element of a wavefront-uniform base plus a per-lane byte offset (a global
 // access with a scalar base address and one vector offset register)

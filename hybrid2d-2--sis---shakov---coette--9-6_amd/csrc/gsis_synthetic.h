@@ -5,22 +5,48 @@
 
 namespace gsis {
 
+// Per-cell inputs of the LU-SGS sweeps of the cells i, j in [3, n-3], written
+// by the flux workgroups of the inner loop: field k of each cell.  Stored by
+// anti-diagonal plane p = i+j (the sweep order) and by the 64 rows of a sweep
+// wavefront: block ((p-6)*nw + (j-3)/64) holds field k of row j at
+// k*64 + (j-3)%64, so a wavefront reads each field of a plane with one
+// contiguous load.
+enum NsRec {
+  kRecU = 0,     // U(1:4) before this iteration's update
+  kRecCbi = 4,   // Lci/2 + Lvi: the weight of dU in the i-direction off-diagonal terms
+  kRecCbj = 5,   // Lcj/2 + Lvj
+  kRecHsi = 6,   // si(i,j)/2
+  kRecHsj = 7,   // sj(i,j)/2
+  kRecRa = 8,    // 1/alfa, the inverse diagonal
+  kRecRes = 9,   // Res(1:4)
+  kRecNi = 13,   // ni1(i,j), ni2(i,j)
+  kRecNj = 15,   // nj1(i,j), nj2(i,j)
+  kNRec = 17
+};
+
 struct SyntheticState {
   bool on = false;
   double tol = 0.0;
   int maxit = 100000;
   int local_dt = 1, if_viscous = 1;
   double cfl = 20.0, dtmax = 10.0, dtmin = 1e-7;
-  double *s00 = nullptr;               // [4][ncell]  S00_GKUA(1:4)
-  double *res = nullptr, *du = nullptr; // [4][ncell]  B%Res, B%dU
-  double *dtl = nullptr, *lci = nullptr, *lcj = nullptr, *lvi = nullptr, *lvj = nullptr;  // [ncell]
-  double *fi = nullptr, *fj = nullptr; // [4][ncell]  face fluxes of Residual_FVM
-  int *iters = nullptr;                // [0]: inner iterations of the last step, [1]: T<0.1 flag
-  void *block = nullptr;               // single allocation holding all of the above
+  int rs = 0, np = 0;        // sweep rows (ny-5) and planes (nx+ny-11); 0 when no cell is swept
+  int nw = 0;                // sweep wavefronts holding rows, ceil(rs/64)
+  int ntx = 0, nty = 0;      // flux tiles per direction
+  int nflux = 0;             // flux workgroups of the inner-loop launch
+  int block = 256;           // threads per workgroup of the inner-loop launch
+  double *s00 = nullptr;     // [4][ncell]  S00_GKUA(1:4)
+  double *rec = nullptr;     // [np][nw][kNRec][64] sweep inputs, see NsRec
+  double *dstar = nullptr;   // [np][nw][4][64] dU after the forward sweep
+  double *part = nullptr;    // [nflux][8]: per flux workgroup {iteration done (int), T<0.1 (int), max |Res(4)|}
+  double *dump = nullptr;    // [4][64] sink of the sweep's stores for lanes without a cell
+  int *ctl = nullptr;        // [1] go (next iteration or -1), [3] a wait timed out
+  int *iters = nullptr;      // [0]: inner iterations of the last step, [1]: T<0.1 flag
+  void *mem = nullptr;       // single allocation holding all of the above
 };
 
 int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st);
-int synthetic_step(const Dev &d, SyntheticState &st, hipStream_t s, int *iters);
+int synthetic_step(const Dev &d, SyntheticState &st, hipStream_t s);
 int synthetic_read_iters(SyntheticState &st, hipStream_t s, int *iters, int *bad_t);
 void synthetic_free(SyntheticState &st);
 const char *synthetic_error();

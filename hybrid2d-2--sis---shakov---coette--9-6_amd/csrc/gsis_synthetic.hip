@@ -10,11 +10,26 @@
 // (src/sub_NS_singlegrid.f90:160-221) and the plane-by-plane LU-SGS
 // (src/sub_LU_SGS.f90:4-143).
 //
-// The O(cells) inner loop is a chain of dependent sweeps, so it runs as one
-// persistent workgroup (1024 threads) that iterates on the device until the
-// max |energy residual| drops to ns_tol or ns_maxit is reached -- every
-// iteration of the loop ends, so the kernel always drains.  The prepare and
-// feedback stages are ordinary grid-wide kernels.
+// The inner loop is one cooperative launch that iterates on the device until
+// the max |energy residual| drops to ns_tol or ns_maxit is reached:
+//  * flux workgroups (1..nflux) own 16x16-cell tiles of the swept cells
+//    3..n-3.  Per iteration each stages U of its tile plus two halo rings in
+//    LDS, forms the primitives and viscosity, the face fluxes (inviscid,
+//    viscous, HoT), then per cell Res, Lc, Lv, dt and everything the LU-SGS
+//    sweeps need that does not depend on dU, stored by anti-diagonal plane;
+//  * workgroup 0 runs the two sweeps.  Lane j of its wavefronts owns row j,
+//    so one sweep step is one anti-diagonal plane: the forward recursion takes
+//    dU(i-1,j) from the lane's own previous step and dU(i,j-1) from the lane
+//    below (a cross-lane shift, through LDS between wavefronts), with one
+//    workgroup barrier per plane.  The backward pass writes U + dU of each
+//    cell as soon as it is final; the sweeps read the pre-update U from the
+//    records, so the update is the reference's "after both sweeps".
+// Any order of the cells that respects the sweep dependences gives the same
+// dU, and the planes are visited in the reference's order, so the inner
+// iteration counts are the reference's.  The loop ends by the reference's
+// rule: iteration k+1 runs when max |Res_4| of iteration k exceeds ns_tol and
+// k < ns_maxit.  Every wait between workgroups is bounded; a timeout ends the
+// loop and is reported by the host.
 #include <string>
 
 #include "gsis_ns_common.h"
@@ -28,10 +43,40 @@ std::string g_syn_err;
 
 const char *synthetic_error() { return g_syn_err.c_str(); }
 
+constexpr int kTile = 16;                    // flux tile: kTile x kTile cells
+constexpr int kTW = kTile + 4;               // with two halo rings each side
+constexpr int kTC = kTW * kTW;
+constexpr int kNFace = (kTile + 1) * kTile;  // i-faces (or j-faces) of a tile
+constexpr int kMaxSweepWaves = 16;           // rows <= 1024
+constexpr long long kWaitTicks = 200000000;  // bounded waits: 2 s of the 100 MHz wall clock
+
+enum { kCtlGo = 1, kCtlErr = 3 };
+
+// Sweep records are blocked by (plane, wavefront): block ((p-6)*nw + w) holds
+// field k of the 64 rows 3+64w .. 3+64w+63 at k*64 + lane, so each field of a
+// plane is one contiguous 512 B load per wavefront.
+__device__ __forceinline__ size_t rec_blk(int p, int w, int nw) { return ((size_t)(p - 6) * nw + w) * (kNRec * 64); }
+__device__ __forceinline__ size_t rec_at(int i, int j, int k, int nw) {
+  const int r = j - 3;
+  return rec_blk(i + j, r >> 6, nw) + k * 64 + (r & 63);
+}
+__device__ __forceinline__ size_t dstar_at(int p, int w, int nw) { return ((size_t)(p - 6) * nw + w) * 256; }
+
 // S0 <- S - S0, S00 <- S on interior cells; the kinetic increment added to the
-// synthetic state of the previous step on cells 3..n-3 (src/Hybrid_Sub.f90:45-94)
+// synthetic state of the previous step on cells 3..n-3 (src/Hybrid_Sub.f90:45-94).
+// Also resets the inner loop's control words.
 __global__ void k_ns_prepare(Dev d, SyntheticState st) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    st.ctl[kCtlGo] = (1.0 > st.tol && st.maxit > 0) ? 1 : -1;  // the reference starts from res = 1
+    st.ctl[kCtlErr] = 0;
+    st.iters[0] = 0;
+    st.iters[1] = 0;
+  }
+  if (t < st.nflux) {  // per flux workgroup: no iteration done, no T<0.1
+    reinterpret_cast<int *>(st.part + 8 * t)[0] = 0;
+    reinterpret_cast<int *>(st.part + 8 * t)[1] = 0;
+  }
   const int nc = (d.nx - 1) * (d.ny - 1);
   if (t >= nc) return;
   const int i = t % (d.nx - 1) + 1, j = t / (d.nx - 1) + 1;
@@ -95,15 +140,67 @@ __device__ void van_leer(const double ql[4], const double qr[4], double g, doubl
   for (int m = 0; m < 4; m++) f[m] = fp[m] + fm[m];
 }
 
-// Residual_FVM face flux (src/sub_Residual.f90:68-202, :210-338)
-__device__ void ns_face_flux(const Dev &d, int dir, int i, int j, bool visc, double fl[4]) {
-  const int N = d.ncell;
+// comput_DFn (src/sub_LU_SGS.f90:122-143): F.n(U + dU) - F.n(U) on the unit
+// normal (n1, n2), each half formed as the reference forms it.  1/rho of each
+// state is shared by its normal velocity and pressure (hardware reciprocal
+// and two Newton steps, about 1 ulp).
+__device__ __forceinline__ double rcp_nr(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = fma(-a, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-a, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ void dfn(const double u[4], const double du[4], double n1, double n2, double g,
+                                    double df[4]) {
+  double u2[4];
+  for (int m = 0; m < 4; m++) u2[m] = u[m] + du[m];
+  const double r1 = rcp_nr(u[0]), r2 = rcp_nr(u2[0]);
+  const double un1 = (u[1] * n1 + u[2] * n2) * r1;
+  const double p1 = (g - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) * r1);
+  const double un2 = (u2[1] * n1 + u2[2] * n2) * r2;
+  const double p2 = (g - 1.0) * (u2[3] - 0.5 * (u2[1] * u2[1] + u2[2] * u2[2]) * r2);
+  df[0] = u2[0] * un2 - u[0] * un1;
+  df[1] = (u2[1] * un2 + p2 * n1) - (u[1] * un1 + p1 * n1);
+  df[2] = (u2[2] * un2 + p2 * n2) - (u[2] * un1 + p1 * n2);
+  df[3] = (u2[3] + p2) * un2 - (u[3] + p1) * un1;
+}
+
+struct NsShared {
+  double U[4][kTC];    // tile + halo
+  double pr[5][kTC];   // d, u, v, T, p (comput_duvtpckw)
+  double mu[kTC];      // amu_of
+  double fi[4][kNFace];  // i-face fluxes, face (i0+a, j0+b) at b*(kTile+1)+a
+  double fj[4][kNFace];  // j-face fluxes, face (i0+a, j0+b) at b*kTile+a
+  double xw[2][kMaxSweepWaves][10];  // sweep: values handed between wavefronts, by step parity
+  double red[kMaxSweepWaves];
+  int word;
+};
+
+// the tile's cell fields for ns_face_of
+struct TileCells {
+  const NsShared &sh;
+  const Dev &d;
+  int ib, jb;  // cell (ib, jb) is LDS cell 0
+  __device__ int l(int i, int j) const { return (j - jb) * kTW + (i - ib); }
+  __device__ double u(int i, int j) const { return sh.pr[1][l(i, j)]; }
+  __device__ double v(int i, int j) const { return sh.pr[2][l(i, j)]; }
+  __device__ double T(int i, int j) const { return sh.pr[3][l(i, j)]; }
+  __device__ double mu(int i, int j) const { return sh.mu[l(i, j)]; }
+  __device__ double x1(int i, int j) const { return d.x1[cidx(d, i, j)]; }
+  __device__ double y1(int i, int j) const { return d.y1[cidx(d, i, j)]; }
+};
+
+// Residual_FVM face flux (src/sub_Residual.f90:68-202, :210-338) of face
+// (i,j), primitives from the tile in LDS
+__device__ void face_flux(const Dev &d, const TileCells &tc, int dir, int i, int j, bool visc, double fl[4]) {
+  const NsShared &sh = tc.sh;
   double ul[4], ur[4];
   for (int m = 0; m < 4; m++) {
-    const double *q = d.prim + (m < 3 ? m : 4) * N;  // d, u, v, p
+    const double *q = sh.pr[m < 3 ? m : 4];  // d, u, v, p
     double u1, u2, u3, u4;
-    if (dir == 0) { u1 = q[cidx(d, i - 2, j)]; u2 = q[cidx(d, i - 1, j)]; u3 = q[cidx(d, i, j)]; u4 = q[cidx(d, i + 1, j)]; }
-    else { u1 = q[cidx(d, i, j - 2)]; u2 = q[cidx(d, i, j - 1)]; u3 = q[cidx(d, i, j)]; u4 = q[cidx(d, i, j + 1)]; }
+    if (dir == 0) { u1 = q[tc.l(i - 2, j)]; u2 = q[tc.l(i - 1, j)]; u3 = q[tc.l(i, j)]; u4 = q[tc.l(i + 1, j)]; }
+    else { u1 = q[tc.l(i, j - 2)]; u2 = q[tc.l(i, j - 1)]; u3 = q[tc.l(i, j)]; u4 = q[tc.l(i, j + 1)]; }
     ul[m] = u2 + 0.5 * minmod_ns(u2 - u1, u3 - u2);
     ur[m] = u3 - 0.5 * minmod_ns(u3 - u2, u4 - u3);
   }
@@ -121,8 +218,8 @@ __device__ void ns_face_flux(const Dev &d, int dir, int i, int j, bool visc, dou
   fl[3] = -f0[3] * s;
   if (!visc) return;
   double o[8];
-  ns_face(d, dir, i, j, o);
-  const int c = cidx(d, i, j);
+  ns_face_of(tc, d, dir, i, j, o);
+  const int N = d.ncell, c = cidx(d, i, j);
   const double *tg = dir == 0 ? d.taoi : d.taoj;
   const double *qg = dir == 0 ? d.qi : d.qj;
   const double t11 = o[0] + tg[c], t22 = o[1] + tg[N + c], t12 = o[2] + tg[2 * N + c];
@@ -135,190 +232,375 @@ __device__ void ns_face_flux(const Dev &d, int dir, int i, int j, bool visc, dou
   fl[3] = fl[3] + (e1 * n1 + e2 * n2) * s;
 }
 
-// comput_DFn, src/sub_LU_SGS.f90:122-143 (U and dU component-major)
-__device__ void comput_dfn(const double u[4], const double du[4], double n1, double n2, double g, double df[4]) {
-  double u2[4];
-  for (int m = 0; m < 4; m++) u2[m] = u[m] + du[m];
-  const double un1 = (u[1] * n1 + u[2] * n2) / u[0];
-  const double p1 = (g - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0]);
-  const double un2 = (u2[1] * n1 + u2[2] * n2) / u2[0];
-  const double p2 = (g - 1.0) * (u2[3] - 0.5 * (u2[1] * u2[1] + u2[2] * u2[2]) / u2[0]);
-  df[0] = u2[0] * un2 - u[0] * un1;
-  df[1] = (u2[1] * un2 + p2 * n1) - (u[1] * un1 + p1 * n1);
-  df[2] = (u2[2] * un2 + p2 * n2) - (u[2] * un1 + p1 * n2);
-  df[3] = (u2[3] + p2) * un2 - (u[3] + p1) * un1;
-}
-
-__device__ double block_max(double x) {
-  __shared__ double shm[16];
-  for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_down(x, off, 64));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  __syncthreads();
-  if (lane == 0) shm[wid] = x;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = shm[0];
-    for (int w = 1; w < nw; w++) m = fmax(m, shm[w]);
-    shm[0] = m;
-  }
-  __syncthreads();
-  const double r = shm[0];
-  __syncthreads();
-  return r;
-}
-
-__device__ __forceinline__ void ld4(const double *a, int N, int c, double v[4]) {
-  for (int m = 0; m < 4; m++) v[m] = a[m * N + c];
-}
-
-// The inner loop (src/Hybrid_Sub.f90:99-104) as one persistent workgroup.
-__global__ void __launch_bounds__(1024) k_ns_loop(Dev d, SyntheticState st) {
-  const int nx = d.nx, ny = d.ny, N = d.ncell, T = blockDim.x, tid = threadIdx.x;
-  const double g = d.gamma, w = 1.0;
+// One tile of Comput_Residual_one_mesh (src/sub_NS_singlegrid.f90:45-114):
+// comput_duvtpckw with its T<0.1 stop, get_viscous, the face fluxes, Res,
+// comput_Lvc, comput_dt and the LU-SGS diagonal (src/sub_LU_SGS.f90:34-36),
+// stored as sweep records.  Returns this tile's max |Res_4| and T<0.1 flag.
+__device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, int tile, double &rmax, int &bad) {
+  const int nx = d.nx, ny = d.ny, N = d.ncell;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const double g = d.gamma;
   const bool visc = st.if_viscous != 0;
-  int it = 0;
-  double res = 1.0;
-  while (res > st.tol && it < st.maxit) {
-    it++;
-    // comput_duvtpckw over the padded grid, T<0.1 stop on interior cells
-    int bad = 0;
-    for (int c = tid; c < N; c += T) {
-      const double dd = d.U[c], u = d.U[N + c] / dd, v = d.U[2 * N + c] / dd;
-      const double tt = (d.U[3 * N + c] - 0.5 * dd * (u * u + v * v)) / (d.cv * dd);
-      d.prim[c] = dd;
-      d.prim[N + c] = u;
-      d.prim[2 * N + c] = v;
-      d.prim[3 * N + c] = tt;
-      d.prim[4 * N + c] = (1. / (g * d.ma * d.ma)) * dd * tt;
-      const int i = c % d.NI - 1, j = c / d.NI - 1;
-      if (i >= 1 && i <= nx - 1 && j >= 1 && j <= ny - 1 && tt < 0.10000000149011612) bad = 1;
+  const int i0 = 3 + (tile % st.ntx) * kTile, j0 = 3 + (tile / st.ntx) * kTile;
+  const int ilast = min(i0 + kTile - 1, nx - 3), jlast = min(j0 + kTile - 1, ny - 3);
+  const int nci = ilast - i0 + 1, ncj = jlast - j0 + 1;
+  const TileCells tc{sh, d, i0 - 2, j0 - 2};
+  for (int t = tid; t < kTC; t += nt) {
+    const int i = i0 - 2 + t % kTW, j = j0 - 2 + t / kTW;
+    if (i > nx + 1 || j > ny + 1) continue;
+    const int c = cidx(d, i, j);
+    double u[4];
+    for (int m = 0; m < 4; m++) {
+      u[m] = ld_agent(d.U + m * N + c);
+      sh.U[m][t] = u[m];
     }
-    bad = __syncthreads_or(bad);
-    if (bad) {
-      if (tid == 0) st.iters[1] = 1;
-      break;
-    }
-    // face fluxes of Residual_FVM
-    const int nfi = nx * (ny - 1), nfj = (nx - 1) * ny;
-    for (int t = tid; t < nfi + nfj; t += T) {
-      double fl[4];
-      int i, j;
-      if (t < nfi) {
-        i = t % nx + 1; j = t / nx + 1;
-        ns_face_flux(d, 0, i, j, visc, fl);
-        for (int m = 0; m < 4; m++) st.fi[m * N + cidx(d, i, j)] = fl[m];
-      } else {
-        const int u = t - nfi;
-        i = u % (nx - 1) + 1; j = u / (nx - 1) + 1;
-        ns_face_flux(d, 1, i, j, visc, fl);
-        for (int m = 0; m < 4; m++) st.fj[m * N + cidx(d, i, j)] = fl[m];
-      }
-    }
-    __syncthreads();
-    // residual, its max on cells 3..n-3, comput_Lvc and comput_dt
-    double rmax = 0.0;
-    const int ncl = (nx - 1) * (ny - 1);
-    for (int t = tid; t < ncl; t += T) {
-      const int i = t % (nx - 1) + 1, j = t / (nx - 1) + 1;
-      const int c = cidx(d, i, j), ci = cidx(d, i + 1, j), cj = cidx(d, i, j + 1);
-      for (int m = 0; m < 4; m++)
-        st.res[m * N + c] = st.fi[m * N + ci] - st.fi[m * N + c] + st.fj[m * N + cj] - st.fj[m * N + c];
-      if (i >= 3 && i <= nx - 3 && j >= 3 && j <= ny - 3) rmax = fmax(rmax, fabs(st.res[3 * N + c]));
-      const int g0 = gidx(d, i, j), gi = gidx(d, i + 1, j), gj = gidx(d, i, j + 1);
-      const double s0 = d.vol[g0];
-      const double si = 0.5 * (d.si[g0] + d.si[gi]);
-      const double ni1 = 0.5 * (d.ni1[g0] + d.ni1[gi]), ni2 = 0.5 * (d.ni2[g0] + d.ni2[gi]);
-      const double sj = 0.5 * (d.sj[g0] + d.sj[gj]);
-      const double nj1 = 0.5 * (d.nj1[g0] + d.nj1[gj]), nj2 = 0.5 * (d.nj2[g0] + d.nj2[gj]);
-      const double uu = d.prim[N + c], vv = d.prim[2 * N + c];
-      const double uni = uu * ni1 + vv * ni2, unj = uu * nj1 + vv * nj2;
-      const double cc = sqrt(d.prim[3 * N + c]) / d.ma;
-      const double lci = (fabs(uni) + cc) * si, lcj = (fabs(unj) + cc) * sj;
-      const double amu = visc ? amu_of(d, c) : 0.0;
-      const double tmp1 = g / d.prim[c] * (amu / d.pr + 0.0 / 0.9);
-      const double lvi = tmp1 * si * si / s0, lvj = tmp1 * sj * sj / s0;
-      st.lci[c] = lci; st.lcj[c] = lcj; st.lvi[c] = lvi; st.lvj[c] = lvj;
-      double dt;
-      if (!st.local_dt) dt = d.dt;
-      else {
-        dt = st.cfl * d.vol[g0] / (lci + lcj + 1.0 * (lvi + lvj));
-        if (dt > st.dtmax) dt = st.dtmax;
-        if (dt < st.dtmin) dt = st.dtmin;
-      }
-      st.dtl[c] = dt;
-    }
-    res = block_max(rmax);  // includes the barrier that publishes res/L*/dt
-    // du_LU_SGS_2D: forward planes (recursive), then backward planes (recursive)
-    for (int plane = 6; plane <= nx + ny - 6; plane++) {
-      const int jlo = max(3, plane - (nx - 3)), jhi = min(ny - 3, plane - 3);
-      for (int j = jlo + tid; j <= jhi; j += T) {
-        const int i = plane - j;
-        const int c = cidx(d, i, j), g0 = gidx(d, i, j);
-        double alfa = d.vol[g0] / st.dtl[c] + w * (st.lci[c] + st.lcj[c]);
-        if (visc) alfa = alfa + 2.0 * (st.lvi[c] + st.lvj[c]);
-        double dui[4] = {0, 0, 0, 0}, duj[4] = {0, 0, 0, 0}, df[4], uu[4], dd[4];
-        if (i != 3) {
-          const int cm = cidx(d, i - 1, j);
-          ld4(d.U, N, cm, uu); ld4(st.du, N, cm, dd);
-          comput_dfn(uu, dd, d.ni1[g0], d.ni2[g0], g, df);
-          for (int m = 0; m < 4; m++) {
-            dui[m] = 0.5 * (df[m] * d.si[g0] + w * st.lci[cm] * dd[m]);
-            if (visc) dui[m] = dui[m] + st.lvi[cm] * dd[m];
-          }
-        }
-        if (j != 3) {
-          const int cm = cidx(d, i, j - 1);
-          ld4(d.U, N, cm, uu); ld4(st.du, N, cm, dd);
-          comput_dfn(uu, dd, d.nj1[g0], d.nj2[g0], g, df);
-          for (int m = 0; m < 4; m++) {
-            duj[m] = 0.5 * (df[m] * d.sj[g0] + w * st.lcj[cm] * dd[m]);
-            if (visc) duj[m] = duj[m] + st.lvj[cm] * dd[m];
-          }
-        }
-        for (int m = 0; m < 4; m++) st.du[m * N + c] = (st.res[m * N + c] + dui[m] + duj[m]) / alfa;
-      }
-      __syncthreads();
-    }
-    for (int plane = nx + ny - 6; plane >= 6; plane--) {
-      const int jlo = max(3, plane - (nx - 3)), jhi = min(ny - 3, plane - 3);
-      for (int j = jlo + tid; j <= jhi; j += T) {
-        const int i = plane - j;
-        const int c = cidx(d, i, j), g0 = gidx(d, i, j);
-        double alfa = d.vol[g0] / st.dtl[c] + w * (st.lci[c] + st.lcj[c]);
-        if (visc) alfa = alfa + 2.0 * (st.lvi[c] + st.lvj[c]);
-        double dui[4] = {0, 0, 0, 0}, duj[4] = {0, 0, 0, 0}, df[4], uu[4], dd[4];
-        if (i != nx - 3) {
-          const int cp = cidx(d, i + 1, j), gp = gidx(d, i + 1, j);
-          ld4(d.U, N, cp, uu); ld4(st.du, N, cp, dd);
-          comput_dfn(uu, dd, d.ni1[g0], d.ni2[g0], g, df);
-          for (int m = 0; m < 4; m++) {
-            dui[m] = -0.5 * (df[m] * d.si[gp] - w * st.lci[cp] * dd[m]);
-            if (visc) dui[m] = dui[m] + st.lvi[cp] * dd[m];
-          }
-        }
-        if (j != ny - 3) {
-          const int cp = cidx(d, i, j + 1), gp = gidx(d, i, j + 1);
-          ld4(d.U, N, cp, uu); ld4(st.du, N, cp, dd);
-          comput_dfn(uu, dd, d.nj1[g0], d.nj2[g0], g, df);
-          for (int m = 0; m < 4; m++) {
-            duj[m] = -0.5 * (df[m] * d.sj[gp] - w * st.lcj[cp] * dd[m]);
-            if (visc) duj[m] = duj[m] + st.lvj[cp] * dd[m];
-          }
-        }
-        for (int m = 0; m < 4; m++) st.du[m * N + c] = st.du[m * N + c] + (dui[m] + duj[m]) / alfa;
-      }
-      __syncthreads();
-    }
-    // U <- U + dU on cells 3..n-3
-    const int nci = (nx - 5) * (ny - 5);
-    for (int t = tid; t < nci; t += T) {
-      const int i = t % (nx - 5) + 3, j = t / (nx - 5) + 3;
-      const int c = cidx(d, i, j);
-      for (int m = 0; m < 4; m++) d.U[m * N + c] = d.U[m * N + c] + st.du[m * N + c];
-    }
-    __syncthreads();
+    const double dd = u[0], uu = u[1] / dd, vv = u[2] / dd;
+    const double T = (u[3] - 0.5 * dd * (uu * uu + vv * vv)) / (d.cv * dd);
+    sh.pr[0][t] = dd;
+    sh.pr[1][t] = uu;
+    sh.pr[2][t] = vv;
+    sh.pr[3][t] = T;
+    sh.pr[4][t] = (1. / (g * d.ma * d.ma)) * dd * T;
+    sh.mu[t] = visc ? 1. / d.re * pow(T, 0.8100000023841858) : 0.0;
+    if (i >= 1 && i <= nx - 1 && j >= 1 && j <= ny - 1 && T < 0.10000000149011612) bad = 1;
   }
-  if (tid == 0) st.iters[0] = it;
+  __syncthreads();
+  const int nfi = (nci + 1) * ncj, nfj = nci * (ncj + 1);
+  for (int t = tid; t < nfi + nfj; t += nt) {
+    double fl[4];
+    if (t < nfi) {
+      const int a = t % (nci + 1), b = t / (nci + 1);
+      face_flux(d, tc, 0, i0 + a, j0 + b, visc, fl);
+      for (int m = 0; m < 4; m++) sh.fi[m][b * (kTile + 1) + a] = fl[m];
+    } else {
+      const int a = (t - nfi) % nci, b = (t - nfi) / nci;
+      face_flux(d, tc, 1, i0 + a, j0 + b, visc, fl);
+      for (int m = 0; m < 4; m++) sh.fj[m][b * kTile + a] = fl[m];
+    }
+  }
+  __syncthreads();
+  const int nw = st.nw;
+  double *R = st.rec;
+  for (int t = tid; t < nci * ncj; t += nt) {
+    const int a = t % nci, b = t / nci, i = i0 + a, j = j0 + b;
+    double res[4];
+    for (int m = 0; m < 4; m++)
+      res[m] = sh.fi[m][b * (kTile + 1) + a + 1] - sh.fi[m][b * (kTile + 1) + a] + sh.fj[m][(b + 1) * kTile + a] -
+               sh.fj[m][b * kTile + a];
+    rmax = fmax(rmax, fabs(res[3]));
+    // comput_Lvc and comput_dt (src/sub_NS_singlegrid.f90:160-221)
+    const int g0 = gidx(d, i, j), gi = gidx(d, i + 1, j), gj = gidx(d, i, j + 1);
+    const double s0 = d.vol[g0];
+    const double si = 0.5 * (d.si[g0] + d.si[gi]);
+    const double ni1 = 0.5 * (d.ni1[g0] + d.ni1[gi]), ni2 = 0.5 * (d.ni2[g0] + d.ni2[gi]);
+    const double sj = 0.5 * (d.sj[g0] + d.sj[gj]);
+    const double nj1 = 0.5 * (d.nj1[g0] + d.nj1[gj]), nj2 = 0.5 * (d.nj2[g0] + d.nj2[gj]);
+    const int lc = tc.l(i, j);
+    const double uu = sh.pr[1][lc], vv = sh.pr[2][lc];
+    const double uni = uu * ni1 + vv * ni2, unj = uu * nj1 + vv * nj2;
+    const double cc = sqrt(sh.pr[3][lc]) / d.ma;
+    const double lci = (fabs(uni) + cc) * si, lcj = (fabs(unj) + cc) * sj;
+    const double tmp1 = g / sh.pr[0][lc] * (sh.mu[lc] / d.pr + 0.0 / 0.9);
+    const double lvi = visc ? tmp1 * si * si / s0 : 0.0, lvj = visc ? tmp1 * sj * sj / s0 : 0.0;
+    double dt;
+    if (!st.local_dt) dt = d.dt;
+    else {
+      dt = st.cfl * d.vol[g0] / (lci + lcj + 1.0 * (lvi + lvj));
+      if (dt > st.dtmax) dt = st.dtmax;
+      if (dt < st.dtmin) dt = st.dtmin;
+    }
+    double alfa = d.vol[g0] / dt + 1.0 * (lci + lcj);
+    if (visc) alfa = alfa + 2.0 * (lvi + lvj);
+    double uc[4];
+    for (int m = 0; m < 4; m++) {
+      uc[m] = sh.U[m][lc];
+      st_agent(R + rec_at(i, j, kRecU + m, nw), uc[m]);
+      st_agent(R + rec_at(i, j, kRecRes + m, nw), res[m]);
+    }
+    st_agent(R + rec_at(i, j, kRecCbi, nw), 0.5 * lci + lvi);
+    st_agent(R + rec_at(i, j, kRecCbj, nw), 0.5 * lcj + lvj);
+    st_agent(R + rec_at(i, j, kRecHsi, nw), 0.5 * d.si[g0]);
+    st_agent(R + rec_at(i, j, kRecHsj, nw), 0.5 * d.sj[g0]);
+    st_agent(R + rec_at(i, j, kRecRa, nw), 1.0 / alfa);
+    const double n1i = d.ni1[g0], n2i = d.ni2[g0], n1j = d.nj1[g0], n2j = d.nj2[g0];
+    st_agent(R + rec_at(i, j, kRecNi, nw), n1i);
+    st_agent(R + rec_at(i, j, kRecNi + 1, nw), n2i);
+    st_agent(R + rec_at(i, j, kRecNj, nw), n1j);
+    st_agent(R + rec_at(i, j, kRecNj + 1, nw), n2j);
+  }
+  __syncthreads();  // the next tile reuses the LDS
+}
+
+__device__ __forceinline__ void barrier_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// x of lane l-1 (wave_shr:1) or l+1 (wave_shl:1) of the wavefront; 0 in the
+// lane that has no such neighbour
+__device__ __forceinline__ double lane_shift(double x, bool up) {
+  const unsigned long long b = __double_as_longlong(x);
+  int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+  if (up) {
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, false);
+  }
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// One sweep step's record of a lane's own cell (all of it in the forward
+// sweep; the backward sweep leaves res and reads dU* instead).
+struct Rec {
+  double res[4], ra, ni[2], nj[2], hsi, hsj, u[4], cbi, cbj;
+};
+
+// A lane without a cell in the plane reads element 0 of the dump instead, for
+// every field: the wavefront's load then fetches only its active rows' lines
+// plus that one line.
+template <bool FWD>
+__device__ __forceinline__ void load_rec(const double *bp, const double *ds, const double *dump, bool on, int lane,
+                                         Rec &x) {
+  const double *b = on ? bp + lane : dump;
+  const double *e = on ? ds + lane : dump;
+  const int f = on ? 64 : 0;
+  for (int m = 0; m < 4; m++) {
+    x.res[m] = FWD ? ld_agent(b + (kRecRes + m) * f) : ld_agent(e + m * f);
+    x.u[m] = ld_agent(b + (kRecU + m) * f);
+  }
+  x.ra = ld_agent(b + kRecRa * f);
+  for (int m = 0; m < 2; m++) {
+    x.ni[m] = ld_agent(b + (kRecNi + m) * f);
+    x.nj[m] = ld_agent(b + (kRecNj + m) * f);
+  }
+  x.hsi = ld_agent(b + kRecHsi * f);
+  x.hsj = ld_agent(b + kRecHsj * f);
+  x.cbi = ld_agent(b + kRecCbi * f);
+  x.cbj = ld_agent(b + kRecCbj * f);
+}
+
+// What a lane carries from one step to the next: its own cell's U, Cbi,
+// Cbj, Hsi, Hsj and dU.  The lane's next cell is the i-neighbour of this one
+// and the next row's lane reads it as its j-neighbour.
+struct Carry {
+  double u[4], cbi, cbj, hsi, hsj, du[4];
+};
+
+// du_LU_SGS_2D (src/sub_LU_SGS.f90:4-113) and the update of cells 3..n-3.
+// Lane j owns row j (j = 3 + thread index); step p visits plane i + j = p.
+// Each step is straight-line code: every lane issues the same loads (a lane
+// without a cell reads the dump) and stores (a lane without a cell writes to
+// the dump), so the waits for a record loaded three steps earlier do not wait
+// for the later ones.
+template <bool FWD>
+__device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh) {
+  const int nx = d.nx, ny = d.ny, N = d.ncell, nwr = st.nw;
+  const double g = d.gamma;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), nw = blockDim.x >> 6;
+  const int j = 3 + tid;
+  const int pmin = 6, pmax = nx + ny - 6;
+  const int nstep = pmax - pmin + 1, nstep3 = (nstep + 2) / 3 * 3;
+  if (w >= nwr) {  // a wavefront without rows keeps the barrier count
+    for (int s = 0; s < nstep3; s++) barrier_lds();
+    return;
+  }
+  const double *R = st.rec;
+  double *dstar = st.dstar;
+  double *dump = st.dump + lane;
+  auto pof = [&](int s) { return FWD ? pmin + s : pmax - s; };
+  auto act = [&](int s) {
+    const int p = pof(s);
+    return s < nstep && j <= ny - 3 && p - j >= 3 && p - j <= nx - 3;
+  };
+  auto blk = [&](int s) { return R + rec_blk(pof(min(s, nstep - 1)), w, nwr); };
+  auto dsb = [&](int s) { return dstar + dstar_at(pof(min(s, nstep - 1)), w, nwr); };
+  Carry c;
+  for (int m = 0; m < 4; m++) { c.u[m] = 1.0; c.du[m] = 0.0; }
+  c.cbi = c.cbj = c.hsi = c.hsj = 0.0;
+  Rec r0, r1, r2;
+  load_rec<FWD>(blk(0), dsb(0), st.dump, act(0), lane, r0);
+  load_rec<FWD>(blk(1), dsb(1), st.dump, act(1), lane, r1);
+  load_rec<FWD>(blk(2), dsb(2), st.dump, act(2), lane, r2);
+  auto step = [&](int s, Rec &x) {
+    const int p = pof(s), i = p - j;
+    const bool on = act(s);
+    // the neighbouring row's values from the previous step: the lane below
+    // (forward) or above (backward), through LDS across wavefronts
+    const int par = (s + 1) & 1;
+    double nu[4], ncb, nhs, ndu[4];
+    for (int m = 0; m < 4; m++) {
+      nu[m] = lane_shift(c.u[m], FWD);
+      ndu[m] = lane_shift(c.du[m], FWD);
+    }
+    ncb = lane_shift(c.cbj, FWD);
+    nhs = lane_shift(c.hsj, FWD);
+    const bool edge = FWD ? (lane == 0 && w > 0) : (lane == 63 && w < nwr - 1);
+    if (edge) {
+      const double *xb = sh.xw[par][FWD ? w - 1 : w + 1];
+      for (int m = 0; m < 4; m++) { nu[m] = xb[m]; ndu[m] = xb[4 + m]; }
+      ncb = xb[8];
+      nhs = xb[9];
+    }
+    // off-diagonal terms from the i-neighbour (this lane's previous cell) and
+    // the j-neighbour
+    double dfi[4], dfj[4], acc[4];
+    dfn(c.u, c.du, x.ni[0], x.ni[1], g, dfi);
+    dfn(nu, ndu, x.nj[0], x.nj[1], g, dfj);
+    const bool usei = FWD ? i != 3 : i != nx - 3, usej = FWD ? j != 3 : j != ny - 3;
+    double du[4];
+    for (int m = 0; m < 4; m++) {
+      // forward:  dui = 0.5 (DF si(i,j) + Lci(i-1,j) dU) + Lvi(i-1,j) dU      (:44-49)
+      // backward: dui = -0.5 (DF si(i+1,j) - Lci(i+1,j) dU) + Lvi(i+1,j) dU   (:82-86)
+      const double ti = FWD ? fma(x.hsi, dfi[m], c.cbi * c.du[m]) : fma(-c.hsi, dfi[m], c.cbi * c.du[m]);
+      const double tj = FWD ? fma(x.hsj, dfj[m], ncb * ndu[m]) : fma(-nhs, dfj[m], ncb * ndu[m]);
+      double a = FWD ? x.res[m] : 0.0;
+      a = usei ? a + ti : a;
+      a = usej ? a + tj : a;
+      acc[m] = a;
+      du[m] = FWD ? acc[m] * x.ra : x.res[m] + acc[m] * x.ra;  // backward: x.res holds dU*
+    }
+    if (FWD) {
+      double *o = on ? dstar + dstar_at(p, w, nwr) + lane : dump;
+      for (int m = 0; m < 4; m++) st_agent(o + m * 64, du[m]);
+    } else {
+      for (int m = 0; m < 4; m++) {
+        double *o = on ? d.U + m * N + cidx(d, i, j) : dump + m * 64;
+        st_agent(o, x.u[m] + du[m]);
+      }
+    }
+    for (int m = 0; m < 4; m++) {
+      c.u[m] = x.u[m];
+      c.du[m] = on ? du[m] : 0.0;
+    }
+    c.cbi = x.cbi;
+    c.cbj = x.cbj;
+    c.hsi = x.hsi;
+    c.hsj = x.hsj;
+    if (FWD ? lane == 63 : lane == 0) {
+      double *xb = sh.xw[s & 1][w];
+      for (int m = 0; m < 4; m++) { xb[m] = c.u[m]; xb[4 + m] = c.du[m]; }
+      xb[8] = c.cbj;
+      xb[9] = c.hsj;
+    }
+    load_rec<FWD>(blk(s + 3), dsb(s + 3), st.dump, act(s + 3), lane, x);
+    barrier_lds();
+  };
+  for (int s = 0; s < nstep3; s += 3) {
+    step(s, r0);
+    step(s + 1, r1);
+    step(s + 2, r2);
+  }
+}
+
+__device__ void sweep(const Dev &d, const SyntheticState &st, NsShared &sh) {
+  sweep_pass<true>(d, st, sh);  // dU* = (Res + dui + duj)/alfa, planes 6 .. nx+ny-6 (:26-69)
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this lane's dU* stores are complete
+  barrier_lds();
+  sweep_pass<false>(d, st, sh);  // dU = dU* + (dui + duj)/alfa, planes nx+ny-6 .. 6 (:73-111), then U + dU
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the U stores are complete
+}
+
+// Polls *p until it reaches want (ge) or equals want or is negative (!ge).
+// Bounded by kWaitTicks of the 100 MHz wall clock; a timeout sets *err.  The
+// flux workgroups wait through a whole sweep, so they poll with long sleeps
+// (SLEEP) to keep their loads off the memory the sweep reads.
+template <int SLEEP>
+__device__ bool wait_until(const int *p, int want, bool ge, int *err) {
+  const long long t0 = wall_clock64();
+  int it = 0, v = ld_agent(p);
+  while (!(ge ? v >= want : (v == want || v < 0))) {
+    __builtin_amdgcn_s_sleep(SLEEP);
+    if ((++it & 15) == 0) {
+      if (ld_agent(err) != 0) return false;
+      if (wall_clock64() - t0 > kWaitTicks) {
+        st_agent(err, 1);
+        return false;
+      }
+    }
+    v = ld_agent(p);
+  }
+  return true;
+}
+
+// The inner loop (src/Hybrid_Sub.f90:99-104): workgroup 0 sweeps, the others
+// compute the fluxes; see the file comment.  Flux workgroup w reports an
+// iteration in its own slot part[8w..]: its max |Res_4| and T<0.1 flag, then
+// the iteration number once those stores are complete.
+template <int B>
+__global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
+  __shared__ NsShared sh;
+  int *ctl = st.ctl;
+  const int tid = threadIdx.x;
+  if (blockIdx.x > 0) {
+    const int wg = blockIdx.x - 1, ntiles = st.ntx * st.nty;
+    int *slot = reinterpret_cast<int *>(st.part + 8 * wg);
+    for (int k = 1;; k++) {
+      if (tid == 0) {
+        int go = -1;
+        if (wait_until<60>(ctl + kCtlGo, k, false, ctl + kCtlErr)) go = ld_agent(ctl + kCtlGo);
+        sh.word = go == k ? 1 : 0;
+      }
+      __syncthreads();
+      const int run = sh.word;
+      __syncthreads();
+      if (!run) return;
+      double rmax = 0.0;
+      int bad = 0;
+      for (int tile = wg; tile < ntiles; tile += st.nflux) flux_tile(d, st, sh, tile, rmax, bad);
+      // the workgroup's max |Res_4| and T<0.1 flag
+      for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_down(rmax, off, 64));
+      bad = __syncthreads_or(bad);
+      if ((tid & 63) == 0) sh.red[tid >> 6] = rmax;
+      __syncthreads();
+      if (tid == 0) {
+        double m = sh.red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = fmax(m, sh.red[q]);
+        st_agent(st.part + 8 * wg + 1, m);
+        st_agent(slot + 1, bad);
+      }
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores are complete
+      __syncthreads();
+      if (tid == 0) st_agent(slot, k);
+    }
+  }
+  // workgroup 0: the LU-SGS sweeps and the loop control
+  int it = 0, bad = 0;
+  if (ld_agent(ctl + kCtlGo) == 1) {
+    for (int k = 1;; k++) {
+      // every flux workgroup's slot holds k; one lane per workgroup polls
+      int ok = 1;
+      double rmax = 0.0;
+      int b = 0;
+      for (int q = tid; q < st.nflux; q += blockDim.x) {
+        const int *slot = reinterpret_cast<const int *>(st.part + 8 * q);
+        if (!wait_until<1>(slot, k, true, ctl + kCtlErr)) ok = 0;
+        rmax = fmax(rmax, ld_agent(st.part + 8 * q + 1));
+        b |= ld_agent(slot + 1);
+      }
+      ok = __syncthreads_and(ok);
+      b = __syncthreads_or(b);
+      if (!ok) break;  // a wait timed out: end the loop
+      for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_down(rmax, off, 64));
+      if ((tid & 63) == 0) sh.red[tid >> 6] = rmax;
+      __syncthreads();
+      double res = sh.red[0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); q++) res = fmax(res, sh.red[q]);
+      bad = b;
+      it = k;
+      __syncthreads();
+      if (bad) break;
+      sweep(d, st, sh);
+      __syncthreads();
+      const bool more = res > st.tol && k < st.maxit;
+      if (tid == 0) st_agent(ctl + kCtlGo, more ? k + 1 : -1);
+      if (!more) break;
+    }
+  }
+  if (tid == 0) {
+    st_agent(ctl + kCtlGo, -1);  // releases the flux workgroups on every exit path
+    st.iters[0] = it;
+    st.iters[1] = bad;
+  }
 }
 
 // U0 <- U and the synthetic correction of S on cells 3..n-3 (src/Hybrid_Sub.f90:108-153)
@@ -346,27 +628,60 @@ __global__ void k_ns_feedback(Dev d, SyntheticState st) {
   }
 }
 
+// the degenerate loop of a mesh without swept cells: max |Res_4| over no
+// cells is 0, so the reference runs one iteration that changes nothing
+__global__ void k_ns_empty(SyntheticState st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) st.iters[0] = st.ctl[kCtlGo] == 1 ? 1 : 0;
+}
+
 int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st) {
   const size_t N = d.ncell;
-  const size_t n_dbl = 4 * N * 5 + 5 * N;
+  st.rs = d.ny - 5 > 0 && d.nx - 5 > 0 ? d.ny - 5 : 0;
+  st.np = st.rs > 0 ? d.nx + d.ny - 11 : 0;
+  if (st.rs > kMaxSweepWaves * 64) {
+    g_syn_err = "synthetic solve: at most " + std::to_string(kMaxSweepWaves * 64 + 5) + " nodes in j";
+    return GSIS_EUNSUPPORTED;
+  }
+  st.ntx = st.rs > 0 ? (d.nx - 5 + kTile - 1) / kTile : 0;
+  st.nty = st.rs > 0 ? (st.rs + kTile - 1) / kTile : 0;
+  const int ntiles = st.ntx * st.nty;
+  st.block = std::max(256, 64 * ((st.rs + 63) / 64));
+  if (ntiles > 0) {
+    int dev = 0, ncu = 0, per = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, st.block <= 256 ? k_ns_loop<256> : k_ns_loop<1024>,
+                                                       st.block, 0);
+    if (e != hipSuccess || per < 1) {
+      g_syn_err = std::string("synthetic solve: occupancy query: ") + hipGetErrorString(e) + ", " +
+                  std::to_string(per) + " workgroups of " + std::to_string(st.block) + " threads per CU";
+      return GSIS_EHIP;
+    }
+    st.nflux = std::min(ntiles, ncu * per - 1);
+    if (st.nflux < 1) {
+      g_syn_err = "synthetic solve: the inner-loop launch does not fit the device";
+      return GSIS_EUNSUPPORTED;
+    }
+  }
+  st.nw = (st.rs + 63) / 64;
+  const size_t nq = (size_t)st.np * st.nw * 64;  // cells of the blocked sweep layout
+  const size_t n_dbl = 4 * N + (size_t)kNRec * nq + 4 * nq + 8 * (size_t)std::max(st.nflux, 1) + 256;
+  const size_t bytes = n_dbl * 8 + 64;
   void *mem = nullptr;
-  if (hipMalloc(&mem, n_dbl * 8 + 64) != hipSuccess || hipMemset(mem, 0, n_dbl * 8 + 64) != hipSuccess) {
+  if (hipMalloc(&mem, bytes) != hipSuccess || hipMemset(mem, 0, bytes) != hipSuccess) {
     g_syn_err = "synthetic solve: device allocation failed";
     return GSIS_EHIP;
   }
   double *b = reinterpret_cast<double *>(mem);
   st.s00 = b; b += 4 * N;
-  st.res = b; b += 4 * N;
-  st.du = b; b += 4 * N;
-  st.fi = b; b += 4 * N;
-  st.fj = b; b += 4 * N;
-  st.dtl = b; b += N;
-  st.lci = b; b += N;
-  st.lcj = b; b += N;
-  st.lvi = b; b += N;
-  st.lvj = b; b += N;
-  st.iters = reinterpret_cast<int *>(b);
-  st.block = mem;
+  st.rec = b; b += kNRec * nq;
+  st.dstar = b; b += 4 * nq;
+  st.part = b; b += 8 * (size_t)std::max(st.nflux, 1);
+  st.dump = b; b += 256;
+  st.ctl = reinterpret_cast<int *>(b);
+  st.iters = st.ctl + 8;
+  st.mem = mem;
   st.tol = p.ns_tol;
   st.maxit = p.ns_maxit > 0 ? p.ns_maxit : 100000;
   st.local_dt = p.local_dt;
@@ -375,43 +690,60 @@ int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st) {
   st.dtmax = p.dtmax;
   st.dtmin = p.dtmin;
   st.on = true;
-  (void)d;
   return 0;
 }
 
-int synthetic_step(const Dev &d, SyntheticState &st, hipStream_t s, int *iters) {
+int synthetic_step(const Dev &d, SyntheticState &st, hipStream_t s) {
   if (!st.on) {
     g_syn_err = "synthetic solve not initialised";
     return GSIS_EARG;
   }
   const int nc = (d.nx - 1) * (d.ny - 1), nci = (d.nx - 5) * (d.ny - 5);
-  hipLaunchKernelGGL(k_ns_prepare, dim3((nc + 255) / 256), dim3(256), 0, s, d, st);
-  hipLaunchKernelGGL(k_ns_loop, dim3(1), dim3(1024), 0, s, d, st);
+  hipLaunchKernelGGL(k_ns_prepare, dim3((std::max(nc, st.nflux) + 255) / 256), dim3(256), 0, s, d, st);
+  if (st.nflux > 0) {
+    Dev dd = d;
+    SyntheticState ss = st;
+    void *args[] = {&dd, &ss};
+    const void *fn = st.block <= 256 ? reinterpret_cast<const void *>(k_ns_loop<256>)
+                                      : reinterpret_cast<const void *>(k_ns_loop<1024>);
+    const hipError_t e = hipLaunchCooperativeKernel(fn,
+                                                    dim3(1 + st.nflux), dim3(st.block), args, 0, s);
+    if (e != hipSuccess) {
+      g_syn_err = std::string("synthetic solve: cooperative launch failed: ") + hipGetErrorString(e);
+      return GSIS_EHIP;
+    }
+  } else {
+    hipLaunchKernelGGL(k_ns_empty, dim3(1), dim3(64), 0, s, st);
+  }
   if (nci > 0) hipLaunchKernelGGL(k_ns_feedback, dim3((nci + 255) / 256), dim3(256), 0, s, d, st);
-  (void)iters;
   return 0;
 }
 
 int synthetic_read_iters(SyntheticState &st, hipStream_t s, int *iters, int *bad_t) {
-  int h[2] = {0, 0};
+  int h[2] = {0, 0}, c[4] = {0, 0, 0, 0};
   if (!st.on) {
     if (iters) *iters = 0;
     if (bad_t) *bad_t = 0;
     return 0;
   }
   if (hipMemcpyAsync(h, st.iters, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(c, st.ctl, sizeof(c), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
     g_syn_err = "synthetic solve: reading the iteration counter failed";
     return GSIS_EHIP;
   }
   if (iters) *iters = h[0];
   if (bad_t) *bad_t = h[1];
+  if (c[kCtlErr]) {
+    g_syn_err = "synthetic solve: a wait between the inner loop's workgroups timed out";
+    return GSIS_EHIP;
+  }
   return 0;
 }
 
 void synthetic_free(SyntheticState &st) {
-  if (st.block) (void)hipFree(st.block);
-  st.block = nullptr;
+  if (st.mem) (void)hipFree(st.mem);
+  st.mem = nullptr;
   st.on = false;
 }
 

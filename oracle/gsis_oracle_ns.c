@@ -240,6 +240,32 @@ static double ns_time_advance(orc_t *o, int *err) {
   return resmax4;
 }
 
+/* n inner iterations of NS_Time_advance_LU_SGS from the current U and HoT
+ * fields, without the GSIS block around them; returns the max |Res(4)| of the
+ * last one (a negative value on T<0.1).  For checks of the inner loop. */
+double orc_ns_advance(orc_t *o, int n) {
+  double r = 0.0;
+  for (int k = 0; k < n; k++) {
+    int err = 0;
+    r = ns_time_advance(o, &err);
+    if (err) return -1.0;
+  }
+  return r;
+}
+
+/* the work arrays of the last inner iteration, Fortran layouts; any pointer
+ * may be NULL: res, du (4,-1:nx+1,-1:ny+1); lc = Lci, Lcj, Lvi, Lvj, dt as
+ * five (-1:nx+1,-1:ny+1) arrays */
+void orc_ns_work(const orc_t *o, double *res, double *du, double *lc) {
+  const size_t nc = (size_t)(o->nx + 3) * (o->ny + 3);
+  if (res) memcpy(res, o->res, 4 * nc * sizeof(double));
+  if (du) memcpy(du, o->du, 4 * nc * sizeof(double));
+  if (lc) {
+    const double *a[5] = {o->lci, o->lcj, o->lvi, o->lvj, o->dtl};
+    for (int k = 0; k < 5; k++) memcpy(lc + k * nc, a[k], nc * sizeof(double));
+  }
+}
+
 /* the GSIS block of Hybrid_Time_advance_LU_SGS, src/Hybrid_Sub.f90:45-153 */
 int orc_synthetic_ns(orc_t *o) {
   const int nx = o->nx, ny = o->ny;

@@ -63,6 +63,10 @@ def load():
         lib.orc_step.restype = ctypes.c_int
         lib.orc_residual.argtypes = [vp, dp, dp]
         lib.orc_residual.restype = ctypes.c_int
+        lib.orc_ns_advance.argtypes = [vp, ctypes.c_int]
+        lib.orc_ns_advance.restype = ctypes.c_double
+        lib.orc_ns_work.argtypes = [vp, dp, dp, dp]
+        lib.orc_ns_work.restype = None
         _lib = lib
     return _lib
 
@@ -222,6 +226,23 @@ class OracleSolver:
 
     def convert_ns(self) -> int:
         return self.lib.orc_convert_ns(self.h)
+
+    def ns_advance(self, n: int = 1) -> float:
+        """n inner iterations of the synthetic NS solve on the current U and
+        HoT fields; the max |Res_4| of the last one (negative on T < 0.1)."""
+        return self.lib.orc_ns_advance(self.h, int(n))
+
+    def ns_work(self) -> dict:
+        """Res, dU, Lci, Lcj, Lvi, Lvj and dt of the last inner iteration."""
+        c = self.case
+        sh = (c.nx + 3, c.ny + 3)
+        res = np.zeros((4,) + sh, order="F")
+        du = np.zeros((4,) + sh, order="F")
+        lc = np.zeros((5,) + sh, order="C")
+        self.lib.orc_ns_work(self.h, _dp(res), _dp(du), _dp(lc))
+        # each of the five is a Fortran (-1:nx+1,-1:ny+1) array
+        lc = np.stack([np.asfortranarray(lc[k].ravel().reshape(sh, order="F")) for k in range(5)])
+        return {"res": res, "du": du, "lci": lc[0], "lcj": lc[1], "lvi": lc[2], "lvj": lc[3], "dt": lc[4]}
 
     def step(self, n: int = 1):
         rc = self.lib.orc_step(self.h, int(n))
