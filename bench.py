@@ -1,17 +1,19 @@
 #!/usr/bin/env python
-"""Benchmark of the GSIS kinetic loop on one MI355X.
+"""Benchmark of the GSIS loop on one MI355X.
 
 One "step" is one outer iteration of Hybrid_Time_advance_LU_SGS over the
-whole mesh and velocity grid (kinetic BCs, Shakhov RHS, LU-SGS sweep, moments,
-DETFP, Convert_NS).  Workload: the plain GSIS lid-driven cavity of the
-baseline (128 x 128 uniform cells, 64 x 64 non-uniform velocity grid dense
-near zero on [-6,6]^2, Kn = 0.1, lid 0.15, diffuse walls at T = 1, FP64),
-started from the gas at rest; synthetic data (no files).
+whole mesh and velocity grid: kinetic BCs, Shakhov RHS, LU-SGS sweep, moments,
+DETFP, Convert_NS and, in plain GSIS (mode M1, the default), the synthetic NS
+solve iterated to its tolerance.  Default workload: the production cavity
+(256 x 256 uniform cells, 96 x 96 non-uniform velocity grid dense near zero
+on [-6,6]^2, Kn = 0.05, lid 0.2, diffuse walls at T = 1, FP64), started from
+the gas at rest; synthetic data (no files).
 
 Prints one JSON line: throughput in cell-velocity updates/s (all inputs
 resident in HBM), a roofline entry for the dominant kernel timed with HIP
-events on the library's stream, and a CPU baseline (the CPU oracle on a
-bounded sample of the same workload, on this host's cores).
+events on the library's stream (and for each other kernel of the step), and
+a CPU baseline (the CPU oracle on a bounded sample of the same workload, on
+this host's cores).
 """
 from __future__ import annotations
 
@@ -101,17 +103,34 @@ def make_case(pkg, n, nv, kn, lid, dt):
     return pkg.cases.cavity_case(n, nv, kn=kn, lid=lid, dt=dt, name=f"cavity{n}_v{nv}_kn{kn}")
 
 
-def cpu_baseline(case, steps):
-    """The oracle (CPU restatement, OpenMP over velocities) on `steps` steps of
-    the same case; returns cell-velocity updates/s."""
+def cpu_baseline(case, steps, mode, gpu_its, ns_sample=10):
+    """The oracle (CPU restatement, OpenMP over velocities) on a bounded
+    sample of the same workload, in cell-velocity updates/s.  M0: `steps`
+    whole steps.  M1: one kinetic step (M0) plus `ns_sample` iterations of the
+    synthetic NS inner loop, extrapolated to a GSIS step with the GPU's inner
+    iteration counts of the timed steps (the same algorithm, so the same
+    counts).  Returns (value, seconds spent, description)."""
     orc = _load("gsis_oracle_py", ROOT / "oracle" / "oracle.py")
-    o = orc.OracleSolver(case)
+    o = orc.OracleSolver(case, mode=0)
     o.initialize()
     t0 = time.perf_counter()
     o.step(steps)
-    t1 = time.perf_counter()
+    t_kin = (time.perf_counter() - t0) / steps
+    spent = t_kin * steps
+    if mode == 0:
+        o.close()
+        return case.ncell * case.nv / t_kin, spent, f"{steps} step(s) of the same case on the C oracle"
+    assert o.convert_ns() == 0
+    t0 = time.perf_counter()
+    o.ns_advance(ns_sample)
+    t_it = (time.perf_counter() - t0) / ns_sample
+    spent += t_it * ns_sample
     o.close()
-    return case.ncell * case.nv * steps / (t1 - t0), t1 - t0
+    its = sum(gpu_its) / max(len(gpu_its), 1)
+    return (case.ncell * case.nv / (t_kin + its * t_it), spent,
+            f"C oracle: {steps} kinetic step(s) ({t_kin:.2f} s each) and {ns_sample} synthetic NS iterations "
+            f"({1e3 * t_it:.1f} ms each) of the same case, extrapolated to the GPU's {its:.0f} inner iterations "
+            f"per step")
 
 
 def copy_probe(nbytes):
@@ -131,25 +150,25 @@ def copy_probe(nbytes):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=128, help="cells per side")
-    ap.add_argument("--nv", type=int, default=64, help="velocities per direction")
-    ap.add_argument("--kn", type=float, default=0.1)
-    ap.add_argument("--lid", type=float, default=0.15)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256, help="cells per side")
+    ap.add_argument("--nv", type=int, default=96, help="velocities per direction")
+    ap.add_argument("--kn", type=float, default=0.05)
+    ap.add_argument("--lid", type=float, default=0.2)
     ap.add_argument("--dt", type=float, default=1.0)
+    ap.add_argument("--mode", choices=("m1", "m0"), default="m1",
+                    help="m1: GSIS with the synthetic NS solve in every step; m0: the kinetic step alone")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps of the CPU-baseline sample (0: skip)")
-    ap.add_argument("--kstep-show", type=int, default=50,
-                    help="residual cadence (the reference's Kstep_show; the kinetic RHS is kept every N steps)")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps of the CPU-baseline sample (0: skip)")
     ap.add_argument("--kernel", choices=("auto", "split", "tiles", "strip16"), default="auto",
                     help="kinetic kernels: one-pass column-wavefront kernel (auto: 8 columns per workgroup, "
                          "strip16: 16), separate RHS and sweep kernels, or the one-pass velocity-tile walker")
     ap.add_argument("--scheme", choices=("nnd", "weno3", "up1"), default="nnd",
                     help="kinetic flux scheme (the reference's Iflag_scheme 1, 3, 0)")
     ap.add_argument("--tiles", type=int, default=0, help="velocity tiles per workgroup of the tile walker (0: auto)")
-    ap.add_argument("--converge", type=int, default=1,
-                    help="1: also measure time-to-converge on the same cavity (first-order upwind), 0: skip")
+    ap.add_argument("--converge", type=int, default=0,
+                    help="1: also measure time-to-converge on a cavity (first-order upwind), 0: skip")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy a device buffer of BYTES bytes 5 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -167,19 +186,25 @@ def main():
     case.const["scheme"] = {"nnd": pkg.capi.SCHEME_NND, "weno3": pkg.capi.SCHEME_WENO3,
                             "up1": pkg.capi.SCHEME_UP1}[args.scheme]
     prec = pkg.capi.PREC_FP32 if args.precision == "fp32" else pkg.capi.PREC_FP64
+    mode = pkg.capi.MODE_M1 if args.mode == "m1" else pkg.capi.MODE_M0
     ksel = {"auto": pkg.capi.KERNEL_AUTO, "split": pkg.capi.KERNEL_SPLIT, "tiles": pkg.capi.KERNEL_TILES,
             "strip16": pkg.capi.KERNEL_STRIP16}[args.kernel]
-    s = solver.GsisSolver(case, precision=prec, kstep_show=args.kstep_show, split_kernels=ksel,
-                          walk_tiles=args.tiles)
+    # the kinetic RHS of the last timed step is kept, for the residual field
+    s = solver.GsisSolver(case, precision=prec, mode=mode, kstep_show=args.warmup + args.steps,
+                          split_kernels=ksel, walk_tiles=args.tiles)
     s.initialize()
     print(f"# case {case.name}: {case.ncell} cells x {case.nv} velocities, precision {args.precision}, "
-          f"scheme {args.scheme}", flush=True)
-    s.step(args.warmup)
+          f"scheme {args.scheme}, mode {args.mode}", flush=True)
+    for k in range(args.warmup):
+        s.step(1)
+        print(f"# warmup step {k + 1}: inner iterations {s.counters()[2]}", flush=True)
     s.counters()  # synchronise the library's stream
     s.timing(1)
+    its = []
     t0 = time.perf_counter()
-    s.step(args.steps)
-    s.counters()
+    for k in range(args.steps):
+        s.step(1)
+        its.append(s.counters()[2])  # synchronises: the inner iteration count of this step
     t1 = time.perf_counter()
     ms, nt = s.timing(0)
     res = s.residual()
@@ -187,7 +212,7 @@ def main():
     if args.precision == "fp32":
         # the same steps on the FP64 path (the one checked against the oracle):
         # relative L2 error of rho, u, T, sigma_xy and q of the FP32 sweeps
-        ref = solver.GsisSolver(case, precision=pkg.capi.PREC_FP64, kstep_show=args.kstep_show)
+        ref = solver.GsisSolver(case, precision=pkg.capi.PREC_FP64, mode=mode)
         ref.initialize()
         ref.step(args.warmup + args.steps)
         fp32_err = pkg.fields.field_errors(s.macro()["s"], ref.macro()["s"], case.nx, case.ny)
@@ -197,32 +222,47 @@ def main():
     units = case.ncell * case.nv
     value = units * args.steps / el
     esz = 4 if args.precision == "fp32" else 8
-    # algorithmic HBM bytes per launch of each kinetic kernel, per interior
-    # cell-velocity: the one-pass kernels (k_strip, k_walk) read and write G
-    # and H once (4*esz); the split path's RHS kernel reads F and writes the
-    # RHS (4*esz) and its sweep reads RHS + F and writes F (6*esz); moments
-    # read F over the padded ring.  The whole step's algorithmic figure is 4*esz.
+    # Algorithmic HBM bytes per launch of each kernel.  Kinetic kernels, per
+    # interior cell-velocity: the one-pass kernels (k_strip, k_walk) read and
+    # write G and H once (4*esz); the split path's RHS kernel reads F and
+    # writes the RHS (4*esz), its sweep reads RHS + F and writes F (6*esz);
+    # moments read F over the padded ring (2*esz).  The synthetic NS loop, per
+    # swept cell and inner iteration: read and write the conserved state U
+    # (4 FP64 each way, 64 B); it is bound by its serial LU-SGS recurrence, not
+    # by these bytes.
     fused = args.kernel != "split"
     one = "walk" if args.kernel == "tiles" else "strip"
     names = [one, "moments"] if fused else ["rhs", "sweep", "moments"]
-    tot = [ms[1], ms[2]] if fused else list(ms)
-    per_unit = {"strip": 4 * esz, "walk": 4 * esz, "rhs": 4 * esz, "sweep": 6 * esz, "moments": 2 * esz}
+    tot = [ms[1], ms[2]] if fused else list(ms[:3])
+    if mode == pkg.capi.MODE_M1:
+        names.append("ns_loop")
+        tot.append(ms[3])
     avg_ms = [m / max(nt, 1) for m in tot]
-    k = max(range(len(names)), key=lambda q: avg_ms[q])
     ring_units = (case.nx + 1) * (case.ny + 1) * case.nv
-    bytes_launch = per_unit[names[k]] * (ring_units if names[k] == "moments" else units)
-    achieved = bytes_launch / (avg_ms[k] * 1e-3) / 1e9
+    swept = max(case.nx - 5, 0) * max(case.ny - 5, 0)
+    per_launch = {"strip": 4 * esz * units, "walk": 4 * esz * units, "rhs": 4 * esz * units,
+                  "sweep": 6 * esz * units, "moments": 2 * esz * ring_units,
+                  "ns_loop": 64 * swept * (sum(its) / max(len(its), 1))}
+    rooflines = {}
+    for q, nm in enumerate(names):
+        ach = per_launch[nm] / (avg_ms[q] * 1e-3) / 1e9 if avg_ms[q] > 0 else 0.0
+        rooflines[nm] = {"bytes_per_launch": per_launch[nm], "avg_launch_ms": avg_ms[q], "achieved": ach,
+                         "frac": ach / HBM_PEAK_GBS}
+    k = max(range(len(names)), key=lambda q: avg_ms[q])
+    kin = 0 if fused else 1
     step_alg = 4 * esz * units / (el / args.steps) / 1e9
-    # HBM bytes per launch of the dominant kernel measured by rocprofv3
-    # (FETCH_SIZE / WRITE_SIZE, scaled by the copy-kernel calibration) for this
-    # workload, when a profile of it is committed
-    traffic, traffic_src = None, None
+    # HBM bytes per launch measured by rocprofv3 (FETCH_SIZE / WRITE_SIZE,
+    # scaled by the copy-kernel calibration) for this workload, when committed
+    traffic, traffic_src = {}, None
     tfile = ROOT / "profiles" / f"hbm_traffic_cavity{args.n}_v{args.nv}.json"
     if tfile.exists() and fused and args.scheme == "nnd":
         rec = json.loads(tfile.read_text()).get(f"cavity{args.n}_v{args.nv}_kn{args.kn}", {})
-        kern = rec.get(args.precision, {}).get("k_" + names[k])
-        if kern:
-            traffic, traffic_src = kern["traffic_bytes"], f"{tfile.relative_to(ROOT)} (rocprofv3 FETCH_SIZE+WRITE_SIZE)"
+        for nm in names:
+            kern = rec.get(args.precision, {}).get("k_" + nm)
+            if kern:
+                traffic[nm] = kern["traffic_bytes"]
+                traffic_src = f"{tfile.relative_to(ROOT)} (rocprofv3 FETCH_SIZE+WRITE_SIZE)"
+    dom = names[k]
     out = {
         "metric": "cell·velocity updates/s + time-to-converge vs plain GSIS at 1e-4 rel L2",
         "value": value,
@@ -236,27 +276,37 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32-sweep/f64-moments",
         "data": "synthetic",
-        "config": {"workload": f"plain GSIS lid-driven cavity {args.n}x{args.n} cells, {args.nv}x{args.nv} sinh "
-                               f"velocity grid on [-6,6]^2, Kn={args.kn}, lid {args.lid}, dt={args.dt}, "
-                               f"{args.precision}, {args.scheme.upper()} fluxes, from rest",
-                   "cells": case.ncell, "velocities": case.nv},
-        "roofline": {"bound": "hbm", "kernel": names[k], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms[k],
-                     "kernel_ms": dict(zip(names, avg_ms)),
+        "config": {"workload": f"{'plain GSIS (synthetic NS solve in every step)' if args.mode == 'm1' else 'kinetic step alone (no synthetic solve)'} "
+                               f"lid-driven cavity {args.n}x{args.n} cells, {args.nv}x{args.nv} sinh velocity grid "
+                               f"on [-6,6]^2, Kn={args.kn}, lid {args.lid}, dt={args.dt}, {args.precision}, "
+                               f"{args.scheme.upper()} fluxes, from rest",
+                   "cells": case.ncell, "velocities": case.nv, "mode": args.mode},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": rooflines[dom]["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": rooflines[dom]["frac"], "traffic": traffic.get(dom),
+                     "traffic_source": traffic_src if dom in traffic else None,
+                     "bytes_per_launch": rooflines[dom]["bytes_per_launch"],
+                     "avg_launch_ms": rooflines[dom]["avg_launch_ms"],
+                     "kernels": {nm: dict(rooflines[nm], traffic=traffic.get(nm)) for nm in names},
                      "step_algorithmic_GBs": step_alg, "step_frac": step_alg / HBM_PEAK_GBS},
         "residual": list(map(float, res)),
     }
+    if mode == pkg.capi.MODE_M1:
+        kin_ms = sum(avg_ms[:len(names) - 1])
+        out["synthetic"] = {"inner_iterations": its, "ns_ms_per_step": avg_ms[-1],
+                            "ns_ms_per_inner_iteration": avg_ms[-1] / max(sum(its) / len(its), 1),
+                            "kinetic_ms_per_step": kin_ms,
+                            "kinetic_only_updates_per_s": units / (kin_ms * 1e-3) if kin_ms > 0 else None}
+        out["roofline"]["note"] = ("the synthetic NS loop dominates the GSIS step; it is bound by the serial "
+                                   "anti-diagonal LU-SGS recurrence (one plane after another), not by HBM")
     if fp32_err is not None:
         out["fp32_rel_l2_vs_fp64"] = fp32_err
     if args.converge and args.precision == "fp64" and args.kernel == "auto" and args.scheme == "nnd":
         out["time_to_converge"] = time_to_converge(pkg, solver, args)
     if args.cpu_steps > 0:
         ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        v, secs = cpu_baseline(case, args.cpu_steps)
+        v, secs, desc = cpu_baseline(case, args.cpu_steps, 1 if args.mode == "m1" else 0, its)
         out["cpu_baseline"] = {"value": v, "unit": "cell·velocity updates/s", "cores": ncores, "kind": "port",
-                               "sample": f"{args.cpu_steps} steps of the same case on the C oracle "
-                                         f"({secs:.1f} s, OpenMP {ncores} threads)"}
+                               "sample": f"{desc} ({secs:.1f} s, OpenMP {ncores} threads)"}
     print(json.dumps(out), flush=True)
 
 

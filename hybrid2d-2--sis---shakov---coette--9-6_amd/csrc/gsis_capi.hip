@@ -56,13 +56,14 @@ struct gsis_handle {
   int swall_len = 0;
   double *resbuf = nullptr;
   int kstep = 0;
+  int res_kstep = -1;  // step count after the last step whose kinetic RHS was kept
   double tt = 0.0;
   int ns_iter = 0;
   gsis::SyntheticState syn{};
   // optional kernel timing (gsis_timing): events around the kinetic kernels
   bool timing = false;
-  hipEvent_t ev[6] = {};
-  double t_ms[3] = {0, 0, 0};
+  hipEvent_t ev[8] = {};
+  double t_ms[4] = {0, 0, 0, 0};
   int t_n = 0;
   bool t_pending = false;
 };
@@ -157,10 +158,15 @@ void moments(gsis_handle *h, bool timed = false) {
   gsis::launch_det_wall(h->d, h->wall_list, h->nwall, h->swall, h->s);
 }
 
+// the last timing event of a step: the end of the synthetic solve in M1,
+// else the end of the moments kernel
+hipEvent_t last_event(gsis_handle *h) { return h->ev[h->p.mode == GSIS_MODE_M1 ? 7 : 5]; }
+
 void harvest_timing(gsis_handle *h) {
   if (!h->t_pending) return;
   float ms = 0.f;
-  for (int k = 0; k < 3; k++)
+  const int nt = h->p.mode == GSIS_MODE_M1 ? 4 : 3;
+  for (int k = 0; k < nt; k++)
     if (hipEventElapsedTime(&ms, h->ev[2 * k], h->ev[2 * k + 1]) == hipSuccess) h->t_ms[k] += ms;
   h->t_n += 1;
   h->t_pending = false;
@@ -169,11 +175,12 @@ void harvest_timing(gsis_handle *h) {
 template <typename T>
 void kinetic(gsis_handle *h) {
   if (h->timing) {
-    if (h->t_pending) { (void)hipEventSynchronize(h->ev[5]); harvest_timing(h); }
+    if (h->t_pending) { (void)hipEventSynchronize(last_event(h)); harvest_timing(h); }
     (void)hipEventRecord(h->ev[0], h->s);
   }
   const int show = h->p.kstep_show;
   const int store = (show <= 0 || (h->kstep + 1) % show == 0) ? 1 : 0;
+  if (store) h->res_kstep = h->kstep + 1;
   T *FG = (T *)h->FG, *FH = (T *)h->FH, *RG = (T *)h->RG, *RH = (T *)h->RH;
   const T *slot = (const T *)h->slot;
   const int sel = h->p.split_kernels;
@@ -213,8 +220,10 @@ int one_step(gsis_handle *h) {
   gsis::launch_convert(h->d, h->s);
   if (h->p.if_viscous) gsis::launch_hot(h->d, h->s);
   if (h->p.mode == GSIS_MODE_M1) {
+    if (h->timing) (void)hipEventRecord(h->ev[6], h->s);
     int rc = gsis::synthetic_step(h->d, h->syn, h->s);
     if (rc) return fail(rc, gsis::synthetic_error());
+    if (h->timing) (void)hipEventRecord(h->ev[7], h->s);
   }
   HIPCHK(hipGetLastError());
   h->tt += h->p.dt;
@@ -514,14 +523,14 @@ int gsis_create(const gsis_params *p, gsis_handle **out) {
 int gsis_timing(gsis_handle *h, int32_t enable, double *ms, int32_t *nsteps) {
   if (!h) return fail(GSIS_EARG, "NULL handle");
   if (enable && !h->ev[0])
-    for (int k = 0; k < 6; k++) HIPCHK(hipEventCreate(&h->ev[k]));
+    for (int k = 0; k < 8; k++) HIPCHK(hipEventCreate(&h->ev[k]));
   HIPCHK(hipStreamSynchronize(h->s));
   harvest_timing(h);
-  if (ms) for (int k = 0; k < 3; k++) ms[k] = h->t_ms[k];
+  if (ms) for (int k = 0; k < 4; k++) ms[k] = h->t_ms[k];
   if (nsteps) *nsteps = h->t_n;
   if (enable >= 0) {
     h->timing = enable != 0;
-    for (int k = 0; k < 3; k++) h->t_ms[k] = 0.0;
+    for (int k = 0; k < 4; k++) h->t_ms[k] = 0.0;
     h->t_n = 0;
   }
   return 0;
@@ -530,7 +539,7 @@ int gsis_timing(gsis_handle *h, int32_t enable, double *ms, int32_t *nsteps) {
 int gsis_destroy(gsis_handle *h) {
   if (!h) return 0;
   if (h->s) (void)hipStreamSynchronize(h->s);
-  for (int k = 0; k < 6; k++)
+  for (int k = 0; k < 8; k++)
     if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
   gsis::synthetic_free(h->syn);
   for (void *q : h->allocs) (void)hipFree(q);
@@ -649,6 +658,9 @@ int gsis_step(gsis_handle *h, int32_t nsteps) {
 
 int gsis_residual(gsis_handle *h, double res_rms[4]) {
   if (!h || !res_rms) return fail(GSIS_EARG, "NULL argument");
+  if (h->res_kstep < 0)
+    return fail(GSIS_EARG, "no kinetic RHS kept yet: the first is kept after a step count that is a multiple of "
+                           "kstep_show (" + std::to_string(h->p.kstep_show) + ")");
   gsis::launch_residual(h->d, h->resbuf, h->s);
   double r[8];
   HIPCHK(hipMemcpyAsync(r, h->resbuf, sizeof(r), hipMemcpyDeviceToHost, h->s));

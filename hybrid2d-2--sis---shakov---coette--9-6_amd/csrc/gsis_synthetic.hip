@@ -48,6 +48,7 @@ constexpr int kTW = kTile + 4;               // with two halo rings each side
 constexpr int kTC = kTW * kTW;
 constexpr int kNFace = (kTile + 1) * kTile;  // i-faces (or j-faces) of a tile
 constexpr int kMaxSweepWaves = 16;           // rows <= 1024
+constexpr int kRing = 8;                     // sweep hand-off slots between wavefronts
 constexpr long long kWaitTicks = 200000000;  // bounded waits: 2 s of the 100 MHz wall clock
 
 enum { kCtlGo = 1, kCtlErr = 3 };
@@ -172,7 +173,12 @@ struct NsShared {
   double mu[kTC];      // amu_of
   double fi[4][kNFace];  // i-face fluxes, face (i0+a, j0+b) at b*(kTile+1)+a
   double fj[4][kNFace];  // j-face fluxes, face (i0+a, j0+b) at b*kTile+a
-  double xw[2][kMaxSweepWaves][10];  // sweep: values handed between wavefronts, by step parity
+  // sweep: lane 63 (forward) or lane 0 (backward) hands its row's values of
+  // step s to the neighbouring wavefront in slot s % kRing
+  double xw[kRing][kMaxSweepWaves][10];
+  int prog[kMaxSweepWaves];  // steps a wavefront has handed on
+  int cons[kMaxSweepWaves];  // steps whose hand-off a wavefront has taken
+  int abort;                 // a hand-off wait timed out
   double red[kMaxSweepWaves];
   int word;
 };
@@ -354,28 +360,48 @@ struct Rec {
   double res[4], ra, ni[2], nj[2], hsi, hsj, u[4], cbi, cbj;
 };
 
-// A lane without a cell in the plane reads element 0 of the dump instead, for
-// every field: the wavefront's load then fetches only its active rows' lines
-// plus that one line.
+// Every lane loads its row's slot of the plane, whether or not the row has a
+// cell there (the slot exists for every row of a sweep wavefront), so the
+// loads take one base address and immediate offsets.
 template <bool FWD>
-__device__ __forceinline__ void load_rec(const double *bp, const double *ds, const double *dump, bool on, int lane,
-                                         Rec &x) {
-  const double *b = on ? bp + lane : dump;
-  const double *e = on ? ds + lane : dump;
-  const int f = on ? 64 : 0;
+__device__ __forceinline__ void load_rec(const double *bp, const double *ds, int lane, Rec &x) {
+  const double *b = bp + lane;
+  const double *e = ds + lane;
   for (int m = 0; m < 4; m++) {
-    x.res[m] = FWD ? ld_agent(b + (kRecRes + m) * f) : ld_agent(e + m * f);
-    x.u[m] = ld_agent(b + (kRecU + m) * f);
+    x.res[m] = FWD ? ld_agent(b + (kRecRes + m) * 64) : e[m * 64];  // dU* was stored by this lane
+    x.u[m] = ld_agent(b + (kRecU + m) * 64);
   }
-  x.ra = ld_agent(b + kRecRa * f);
+  x.ra = ld_agent(b + kRecRa * 64);
   for (int m = 0; m < 2; m++) {
-    x.ni[m] = ld_agent(b + (kRecNi + m) * f);
-    x.nj[m] = ld_agent(b + (kRecNj + m) * f);
+    x.ni[m] = ld_agent(b + (kRecNi + m) * 64);
+    x.nj[m] = ld_agent(b + (kRecNj + m) * 64);
   }
-  x.hsi = ld_agent(b + kRecHsi * f);
-  x.hsj = ld_agent(b + kRecHsj * f);
-  x.cbi = ld_agent(b + kRecCbi * f);
-  x.cbj = ld_agent(b + kRecCbj * f);
+  x.hsi = ld_agent(b + kRecHsi * 64);
+  x.hsj = ld_agent(b + kRecHsj * 64);
+  x.cbi = ld_agent(b + kRecCbi * 64);
+  x.cbj = ld_agent(b + kRecCbj * 64);
+}
+
+// LDS words read and written by several wavefronts of the workgroup
+__device__ __forceinline__ int lds_get(int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+__device__ __forceinline__ void lds_put(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Waits until the LDS word *p reaches want; bounded by kWaitTicks, after
+// which (or once another wavefront has timed out) it gives up, sets the
+// workgroup's abort word and the error word, and the sweep runs to its end on
+// whatever values it has, so that every wavefront still reaches the barriers.
+__device__ __forceinline__ void wait_lds(int *p, int want, int *abort, int *err) {
+  if (lds_get(p) >= want || lds_get(abort)) return;
+  const long long t0 = wall_clock64();
+  while (lds_get(p) < want && !lds_get(abort)) {
+    __builtin_amdgcn_s_sleep(1);
+    if (wall_clock64() - t0 > kWaitTicks) {
+      lds_put(abort, 1);
+      st_agent(err, 1);
+    }
+  }
 }
 
 // What a lane carries from one step to the next: its own cell's U, Cbi,
@@ -387,10 +413,11 @@ struct Carry {
 
 // du_LU_SGS_2D (src/sub_LU_SGS.f90:4-113) and the update of cells 3..n-3.
 // Lane j owns row j (j = 3 + thread index); step p visits plane i + j = p.
-// Each step is straight-line code: every lane issues the same loads (a lane
-// without a cell reads the dump) and stores (a lane without a cell writes to
-// the dump), so the waits for a record loaded three steps earlier do not wait
-// for the later ones.
+// Each step is straight-line code: every lane issues the same loads and
+// stores (a lane without a cell stores to the dump), so the waits for a
+// record loaded three steps earlier do not wait for the later ones.  The
+// loop runs a multiple of three steps; the steps after the last plane have no
+// cell in any row.
 template <bool FWD>
 __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh) {
   const int nx = d.nx, ny = d.ny, N = d.ncell, nwr = st.nw;
@@ -400,10 +427,12 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   const int j = 3 + tid;
   const int pmin = 6, pmax = nx + ny - 6;
   const int nstep = pmax - pmin + 1, nstep3 = (nstep + 2) / 3 * 3;
-  if (w >= nwr) {  // a wavefront without rows keeps the barrier count
-    for (int s = 0; s < nstep3; s++) barrier_lds();
-    return;
-  }
+  if (w >= nwr) return;  // a wavefront without rows
+  // the wavefront this one takes the neighbouring row from, and the one it
+  // hands its edge row to
+  const int from = FWD ? w - 1 : w + 1, to = FWD ? w + 1 : w - 1;
+  const bool has_from = from >= 0 && from < nwr, has_to = to >= 0 && to < nwr;
+  int *prog = sh.prog, *cons = sh.cons, *abort = &sh.abort;
   const double *R = st.rec;
   double *dstar = st.dstar;
   double *dump = st.dump + lane;
@@ -418,15 +447,14 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   for (int m = 0; m < 4; m++) { c.u[m] = 1.0; c.du[m] = 0.0; }
   c.cbi = c.cbj = c.hsi = c.hsj = 0.0;
   Rec r0, r1, r2;
-  load_rec<FWD>(blk(0), dsb(0), st.dump, act(0), lane, r0);
-  load_rec<FWD>(blk(1), dsb(1), st.dump, act(1), lane, r1);
-  load_rec<FWD>(blk(2), dsb(2), st.dump, act(2), lane, r2);
+  load_rec<FWD>(blk(0), dsb(0), lane, r0);
+  load_rec<FWD>(blk(1), dsb(1), lane, r1);
+  load_rec<FWD>(blk(2), dsb(2), lane, r2);
   auto step = [&](int s, Rec &x) {
     const int p = pof(s), i = p - j;
     const bool on = act(s);
     // the neighbouring row's values from the previous step: the lane below
     // (forward) or above (backward), through LDS across wavefronts
-    const int par = (s + 1) & 1;
     double nu[4], ncb, nhs, ndu[4];
     for (int m = 0; m < 4; m++) {
       nu[m] = lane_shift(c.u[m], FWD);
@@ -434,12 +462,16 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     }
     ncb = lane_shift(c.cbj, FWD);
     nhs = lane_shift(c.hsj, FWD);
-    const bool edge = FWD ? (lane == 0 && w > 0) : (lane == 63 && w < nwr - 1);
-    if (edge) {
-      const double *xb = sh.xw[par][FWD ? w - 1 : w + 1];
-      for (int m = 0; m < 4; m++) { nu[m] = xb[m]; ndu[m] = xb[4 + m]; }
-      ncb = xb[8];
-      nhs = xb[9];
+    if (has_from && s > 0) {  // the edge row of step s-1 from the neighbouring wavefront
+      wait_lds(prog + from, s, abort, st.ctl + kCtlErr);
+      if (FWD ? lane == 0 : lane == 63) {
+        const double *xb = sh.xw[(s - 1) % kRing][from];
+        for (int m = 0; m < 4; m++) { nu[m] = xb[m]; ndu[m] = xb[4 + m]; }
+        ncb = xb[8];
+        nhs = xb[9];
+      }
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the slot is read
+      if (lane == 0) lds_put(cons + w, s);
     }
     // off-diagonal terms from the i-neighbour (this lane's previous cell) and
     // the j-neighbour
@@ -459,13 +491,15 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       acc[m] = a;
       du[m] = FWD ? acc[m] * x.ra : x.res[m] + acc[m] * x.ra;  // backward: x.res holds dU*
     }
+    // plain stores: dU* is read back by this lane, U is written back to the
+    // device's coherence point by the release fence after the sweep
     if (FWD) {
       double *o = on ? dstar + dstar_at(p, w, nwr) + lane : dump;
-      for (int m = 0; m < 4; m++) st_agent(o + m * 64, du[m]);
+      for (int m = 0; m < 4; m++) o[m * 64] = du[m];
     } else {
       for (int m = 0; m < 4; m++) {
         double *o = on ? d.U + m * N + cidx(d, i, j) : dump + m * 64;
-        st_agent(o, x.u[m] + du[m]);
+        *o = x.u[m] + du[m];
       }
     }
     for (int m = 0; m < 4; m++) {
@@ -476,14 +510,18 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     c.cbj = x.cbj;
     c.hsi = x.hsi;
     c.hsj = x.hsj;
-    if (FWD ? lane == 63 : lane == 0) {
-      double *xb = sh.xw[s & 1][w];
-      for (int m = 0; m < 4; m++) { xb[m] = c.u[m]; xb[4 + m] = c.du[m]; }
-      xb[8] = c.cbj;
-      xb[9] = c.hsj;
+    if (has_to) {  // hand the edge row on; slot s % kRing must have been taken
+      wait_lds(cons + to, s - kRing + 1, abort, st.ctl + kCtlErr);
+      if (FWD ? lane == 63 : lane == 0) {
+        double *xb = sh.xw[s % kRing][w];
+        for (int m = 0; m < 4; m++) { xb[m] = c.u[m]; xb[4 + m] = c.du[m]; }
+        xb[8] = c.cbj;
+        xb[9] = c.hsj;
+      }
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the slot is written
+      if (lane == 0) lds_put(prog + w, s + 1);
     }
-    load_rec<FWD>(blk(s + 3), dsb(s + 3), st.dump, act(s + 3), lane, x);
-    barrier_lds();
+    load_rec<FWD>(blk(s + 3), dsb(s + 3), lane, x);
   };
   for (int s = 0; s < nstep3; s += 3) {
     step(s, r0);
@@ -492,12 +530,22 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   }
 }
 
+// The wavefronts of a pass are coupled only through their hand-offs: a
+// wavefront may run up to kRing steps ahead of the one that takes its edge row.
 __device__ void sweep(const Dev &d, const SyntheticState &st, NsShared &sh) {
+  const int tid = threadIdx.x;
+  if (tid < kMaxSweepWaves) { sh.prog[tid] = 0; sh.cons[tid] = 0; }
+  if (tid == 0) sh.abort = 0;
+  barrier_lds();
   sweep_pass<true>(d, st, sh);  // dU* = (Res + dui + duj)/alfa, planes 6 .. nx+ny-6 (:26-69)
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this lane's dU* stores are complete
   barrier_lds();
+  if (tid < kMaxSweepWaves) { sh.prog[tid] = 0; sh.cons[tid] = 0; }
+  barrier_lds();
   sweep_pass<false>(d, st, sh);  // dU = dU* + (dui + duj)/alfa, planes nx+ny-6 .. 6 (:73-111), then U + dU
-  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the U stores are complete
+  // release: the U stores reach the device's coherence point before the
+  // flux workgroups are told to read U (L2 write-back, then vmcnt(0))
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 }
 
 // Polls *p until it reaches want (ge) or equals want or is negative (!ge).

@@ -142,8 +142,9 @@ class GsisSolver:
         return done, hist
 
     def timing(self, enable: int = -1):
-        """Kernel timings (ms summed over timed steps) of [rhs, sweep, moments]."""
-        ms = np.zeros(3)
+        """Kernel timings (ms summed over timed steps) of [rhs, sweep, moments,
+        synthetic NS solve]."""
+        ms = np.zeros(4)
         n = ctypes.c_int32()
         capi.check(self.lib, self.lib.gsis_timing(self.h, int(enable), _dp(ms), ctypes.byref(n)))
         return ms, n.value

@@ -148,7 +148,8 @@ int gsis_step(gsis_handle *h, int32_t nsteps);
 /* RMS of the four conserved moments of the kinetic RHS of the last step
  * whose RHS was kept (see kstep_show), B%Res_rms(1:4) of DETFP_res
  * (src/GKUA_DETFP_RES.f90:45-54).
- * Returns GSIS_ENAN when res_rms[0] is NaN (src/IO.f90:236-244). */
+ * Returns GSIS_ENAN when res_rms[0] is NaN (src/IO.f90:236-244), and
+ * GSIS_EARG when no step of this handle has kept its RHS yet. */
 int gsis_residual(gsis_handle *h, double res_rms[4]);
 
 /* Download macro state.  Any pointer may be NULL.  Shapes (Fortran):
@@ -204,9 +205,10 @@ int gsis_inner_exchange(gsis_handle *dst, int32_t ksub, gsis_handle *src, int32_
 
 /* Kernel timing with HIP events on the handle's stream.  No counterpart in
  * the reference: it exists for benchmarks.  enable=1 clears and starts the
- * accumulators, 0 clears and stops them, -1 only reads.  ms[0..2] receive the
- * summed durations (ms) of the kinetic RHS kernel, the LU-SGS sweep kernel and
- * the moments kernel over *nsteps timed steps. */
+ * accumulators, 0 clears and stops them, -1 only reads.  ms[0..3] receive the
+ * summed durations (ms) of the kinetic RHS kernel, the LU-SGS sweep kernel
+ * (for the one-pass kernels: the whole kinetic step), the moments kernel and,
+ * in mode M1, the synthetic NS solve over *nsteps timed steps. */
 int gsis_timing(gsis_handle *h, int32_t enable, double *ms, int32_t *nsteps);
 
 int gsis_destroy(gsis_handle *h);

@@ -346,6 +346,23 @@ def test_nan_in_distribution_is_reported():
     cpu.close()
 
 
+def test_residual_before_any_kept_rhs_is_refused():
+    """gsis_residual reports GSIS_EARG until a step has kept its kinetic RHS
+    (the step count after it a multiple of kstep_show), instead of returning
+    zeros that a host loop would read as converged."""
+    g = load_pkg()
+    case = g.cases.cavity_case(8, 8, kn=0.1, lid=0.15, dt=0.5)
+    gpu = solver_mod().GsisSolver(case, kstep_show=4)
+    gpu.initialize()
+    gpu.step(3)
+    with pytest.raises(g.capi.GsisError) as e:
+        gpu.residual()
+    assert e.value.code == g.capi.GSIS_EARG and "kstep_show" in str(e.value)
+    gpu.step(1)
+    assert np.all(np.isfinite(gpu.residual()))
+    gpu.close()
+
+
 def test_residual_cadence_matches_oracle():
     """With kstep_show=10 the RHS is kept only every 10th step; the residual
     history and the iteration count still match the oracle's."""
