@@ -11,7 +11,8 @@ import sys
 import tempfile
 
 PKG = pathlib.Path(__file__).resolve().parent
-SRC = [PKG / "csrc" / "gsis_kernels.hip", PKG / "csrc" / "gsis_capi.hip", PKG / "csrc" / "gsis_synthetic.hip"]
+SRC = [PKG / "csrc" / "gsis_kernels.hip", PKG / "csrc" / "gsis_capi.hip", PKG / "csrc" / "gsis_synthetic.hip",
+       PKG / "csrc" / "gsis_mlp.hip", PKG / "csrc" / "gsis_closure.hip"]
 OUT = PKG / "libgsis.so"
 HIPCC = "/opt/rocm/bin/hipcc"
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

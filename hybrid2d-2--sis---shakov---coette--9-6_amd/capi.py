@@ -59,7 +59,24 @@ EXPORTED = (
     "gsis_timing",
     "gsis_destroy",
     "gsis_last_error",
+    "gsis_closure_samples",
+    "gsis_set_closure",
+    "gsis_closure_stats",
+    "gsis_mlp_create",
+    "gsis_mlp_nparams",
+    "gsis_mlp_set_params",
+    "gsis_mlp_get_params",
+    "gsis_mlp_set_scaling",
+    "gsis_mlp_forward",
+    "gsis_mlp_loss_grad",
+    "gsis_mlp_train",
+    "gsis_mlp_destroy",
 )
+
+MLP_NIN = 8
+MLP_NOUT = 5
+MLP_WIDTH = 128
+MLP_HIDDEN = 4
 
 
 class GsisBC(ctypes.Structure):
@@ -163,6 +180,21 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.gsis_timing.argtypes = [h, ctypes.c_int32, c_dp, ctypes.POINTER(ctypes.c_int32)]
     lib.gsis_destroy.argtypes = [h]
     lib.gsis_last_error.restype = ctypes.c_char_p
+    c_fp = ctypes.POINTER(ctypes.c_float)
+    c_i64p = ctypes.POINTER(ctypes.c_int64)
+    lib.gsis_closure_samples.argtypes = [h, c_fp, c_fp, ctypes.c_int64, c_i64p]
+    lib.gsis_set_closure.argtypes = [h, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]
+    lib.gsis_closure_stats.argtypes = [h, c_i64p, c_i64p, ctypes.POINTER(ctypes.c_int32)]
+    lib.gsis_mlp_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    lib.gsis_mlp_nparams.argtypes = []
+    lib.gsis_mlp_set_params.argtypes = [ctypes.c_void_p, c_fp]
+    lib.gsis_mlp_get_params.argtypes = [ctypes.c_void_p, c_fp]
+    lib.gsis_mlp_set_scaling.argtypes = [ctypes.c_void_p, c_fp]
+    lib.gsis_mlp_forward.argtypes = [ctypes.c_void_p, c_fp, ctypes.c_int32, c_fp]
+    lib.gsis_mlp_loss_grad.argtypes = [ctypes.c_void_p, c_fp, c_fp, ctypes.c_int32, c_dp, c_fp]
+    lib.gsis_mlp_train.argtypes = [ctypes.c_void_p, c_fp, c_fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_float, ctypes.c_uint64, c_dp]
+    lib.gsis_mlp_destroy.argtypes = [ctypes.c_void_p]
     for name in EXPORTED:
         if name != "gsis_last_error":
             getattr(lib, name).restype = ctypes.c_int

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/gsis_capi.h"
+#include "gsis_closure.h"
 #include "gsis_kernels.h"
 #include "gsis_synthetic.h"
 
@@ -29,6 +30,11 @@ int fail(int code, const std::string &msg) {
   return code;
 }
 }  // namespace
+
+namespace gsis {
+// the message gsis_last_error reports, for the entry points of other files
+void set_last_error(const std::string &msg) { g_err = msg; }
+}  // namespace gsis
 
 #define HIPCHK(expr)                                                                          \
   do {                                                                                        \
@@ -60,6 +66,11 @@ struct gsis_handle {
   double tt = 0.0;
   int ns_iter = 0;
   gsis::SyntheticState syn{};
+  // learned HoT closure of the synthetic solve (gsis_set_closure)
+  gsis_mlp *net = nullptr;
+  float cl_gate = 0.0f;
+  int cl_every = 0;
+  gsis::ClosureBufs cl{};
   // optional kernel timing (gsis_timing): events around the kinetic kernels
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -211,6 +222,30 @@ void kinetic(gsis_handle *h) {
   if (h->timing) (void)hipEventRecord(h->ev[3], h->s);
 }
 
+// The synthetic solve with the learned closure: the inner loop in launches of
+// cl_every iterations, the HoT of the accepted closure faces updated from the
+// network between them; the same iterations as one launch otherwise.
+int synthetic_with_closure(gsis_handle *h) {
+  gsis::MlpDev &net = h->net->d;
+  int rc = gsis::synthetic_begin(h->d, h->syn, h->s);
+  if (rc) return fail(rc, gsis::synthetic_error());
+  if (gsis::closure_begin(h->d, h->cl, net, h->cl_gate, h->s)) return fail(GSIS_EHIP, "closure: start failed");
+  for (int k0 = 1;;) {
+    rc = gsis::synthetic_run(h->d, h->syn, k0, k0 + h->cl_every - 1, h->s);
+    if (rc) return fail(rc, gsis::synthetic_error());
+    bool more = false;
+    int it = 0;
+    rc = gsis::synthetic_unfinished(h->syn, h->s, &more, &it);
+    if (rc) return fail(rc, gsis::synthetic_error());
+    if (!more) break;
+    if (gsis::closure_refresh(h->d, h->cl, net, h->s)) return fail(GSIS_EHIP, "closure: HoT update failed");
+    k0 = it + 1;
+  }
+  if (gsis::closure_end(h->d, h->cl, h->s)) return fail(GSIS_EHIP, "closure: restoring the DVM HoT failed");
+  gsis::synthetic_finish(h->d, h->syn, h->s);
+  return 0;
+}
+
 template <typename T>
 int one_step(gsis_handle *h) {
   gsis::launch_save_s0(h->d, h->s);
@@ -221,8 +256,8 @@ int one_step(gsis_handle *h) {
   if (h->p.if_viscous) gsis::launch_hot(h->d, h->s);
   if (h->p.mode == GSIS_MODE_M1) {
     if (h->timing) (void)hipEventRecord(h->ev[6], h->s);
-    int rc = gsis::synthetic_step(h->d, h->syn, h->s);
-    if (rc) return fail(rc, gsis::synthetic_error());
+    int rc = h->net ? synthetic_with_closure(h) : gsis::synthetic_step(h->d, h->syn, h->s);
+    if (rc) return rc;
     if (h->timing) (void)hipEventRecord(h->ev[7], h->s);
   }
   HIPCHK(hipGetLastError());
@@ -520,6 +555,56 @@ int gsis_create(const gsis_params *p, gsis_handle **out) {
   return 0;
 }
 
+int gsis_closure_samples(gsis_handle *h, float *x, float *y, int64_t cap, int64_t *n) {
+  if (!h || !n) return fail(GSIS_EARG, "NULL argument");
+  int nfi = 0;
+  const int nf = gsis::closure_faces(h->d, &nfi);
+  *n = nf;
+  if (!x && !y) return 0;
+  if (cap < nf) return fail(GSIS_EARG, "closure samples: cap is below the number of closure faces");
+  if (nf == 0) return 0;
+  float *dx = nullptr, *dy = nullptr;
+  HIPCHK(hipMalloc(&dx, (size_t)nf * (GSIS_MLP_NIN + GSIS_MLP_NOUT) * sizeof(float)));
+  dy = dx + (size_t)nf * GSIS_MLP_NIN;
+  gsis::launch_cl_features(h->d, nfi, nf, true, dx, h->s);
+  gsis::launch_cl_targets(h->d, nfi, nf, dy, h->s);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && x) e = hipMemcpyAsync(x, dx, (size_t)nf * GSIS_MLP_NIN * 4, hipMemcpyDeviceToHost, h->s);
+  if (e == hipSuccess && y) e = hipMemcpyAsync(y, dy, (size_t)nf * GSIS_MLP_NOUT * 4, hipMemcpyDeviceToHost, h->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+  (void)hipFree(dx);
+  if (e != hipSuccess) return fail(GSIS_EHIP, std::string("closure samples: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int gsis_set_closure(gsis_handle *h, gsis_mlp *net, double gate, int32_t every) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  if (!net) {
+    h->net = nullptr;
+    return 0;
+  }
+  if (h->p.mode != GSIS_MODE_M1) return fail(GSIS_EARG, "the closure needs the synthetic solve (mode M1)");
+  if (every < 1 || !(gate >= 0.0)) return fail(GSIS_EARG, "closure: every >= 1 and gate >= 0 required");
+  if (gsis::closure_alloc(h->d, h->cl)) return fail(GSIS_EHIP, "closure: device allocation failed");
+  h->net = net;
+  h->cl_gate = (float)gate;
+  h->cl_every = every;
+  return 0;
+}
+
+int gsis_closure_stats(gsis_handle *h, int64_t *faces, int64_t *accepted, int32_t *refreshes) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  int acc = 0;
+  if (h->cl.count) {
+    HIPCHK(hipMemcpyAsync(&acc, h->cl.count, sizeof(int), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipStreamSynchronize(h->s));
+  }
+  if (faces) *faces = h->cl.mem ? h->cl.nf : gsis::closure_faces(h->d, nullptr);
+  if (accepted) *accepted = acc;
+  if (refreshes) *refreshes = h->cl.refreshes;
+  return 0;
+}
+
 int gsis_timing(gsis_handle *h, int32_t enable, double *ms, int32_t *nsteps) {
   if (!h) return fail(GSIS_EARG, "NULL handle");
   if (enable && !h->ev[0])
@@ -542,6 +627,7 @@ int gsis_destroy(gsis_handle *h) {
   for (int k = 0; k < 8; k++)
     if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
   gsis::synthetic_free(h->syn);
+  gsis::closure_free(h->cl);
   for (void *q : h->allocs) (void)hipFree(q);
   if (h->s) (void)hipStreamDestroy(h->s);
   delete h;

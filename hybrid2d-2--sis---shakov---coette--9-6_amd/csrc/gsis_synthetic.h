@@ -41,12 +41,26 @@ struct SyntheticState {
   double *part = nullptr;    // [nflux][8]: per flux workgroup {iteration done (int), T<0.1 (int), max |Res(4)|}
   double *dump = nullptr;    // [4][64] sink of the sweep's stores for lanes without a cell
   int *ctl = nullptr;        // [1] go (next iteration or -1), [3] a wait timed out
-  int *iters = nullptr;      // [0]: inner iterations of the last step, [1]: T<0.1 flag
+  int *iters = nullptr;      // [0]: inner iterations of the last step, [1]: T<0.1 flag,
+                             // [2]: the launch stopped at kstop with the loop unfinished
   void *mem = nullptr;       // single allocation holding all of the above
+  int k0 = 1;                // first inner iteration of a launch
+  int kstop = 0x7fffffff;    // last inner iteration a launch may run
 };
 
 int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st);
+// one synthetic solve: synthetic_begin, synthetic_run(1, no limit), synthetic_finish
 int synthetic_step(const Dev &d, SyntheticState &st, hipStream_t s);
+// the parts of a synthetic solve: the kinetic increment into the synthetic
+// state (src/Hybrid_Sub.f90:45-94); inner iterations k0..at most kstop of the
+// loop (src/Hybrid_Sub.f90:99-104), k0 > 1 continuing a launch that stopped
+// at kstop = k0 - 1; the correction of S (src/Hybrid_Sub.f90:108-153)
+int synthetic_begin(const Dev &d, SyntheticState &st, hipStream_t s);
+int synthetic_run(const Dev &d, SyntheticState &st, int k0, int kstop, hipStream_t s);
+void synthetic_finish(const Dev &d, SyntheticState &st, hipStream_t s);
+// after synthetic_run: whether the loop stopped at kstop unfinished, and the
+// inner iterations run so far (synchronises the stream)
+int synthetic_unfinished(SyntheticState &st, hipStream_t s, bool *unfinished, int *iters);
 int synthetic_read_iters(SyntheticState &st, hipStream_t s, int *iters, int *bad_t);
 void synthetic_free(SyntheticState &st);
 const char *synthetic_error();

@@ -73,6 +73,7 @@ __global__ void k_ns_prepare(Dev d, SyntheticState st) {
     st.ctl[kCtlErr] = 0;
     st.iters[0] = 0;
     st.iters[1] = 0;
+    st.iters[2] = 0;
   }
   if (t < st.nflux) {  // per flux workgroup: no iteration done, no T<0.1
     reinterpret_cast<int *>(st.part + 8 * t)[0] = 0;
@@ -582,7 +583,7 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
   if (blockIdx.x > 0) {
     const int wg = blockIdx.x - 1, ntiles = st.ntx * st.nty;
     int *slot = reinterpret_cast<int *>(st.part + 8 * wg);
-    for (int k = 1;; k++) {
+    for (int k = st.k0;; k++) {
       if (tid == 0) {
         int go = -1;
         if (wait_until<60>(ctl + kCtlGo, k, false, ctl + kCtlErr)) go = ld_agent(ctl + kCtlGo);
@@ -612,9 +613,9 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
     }
   }
   // workgroup 0: the LU-SGS sweeps and the loop control
-  int it = 0, bad = 0;
-  if (ld_agent(ctl + kCtlGo) == 1) {
-    for (int k = 1;; k++) {
+  int it = st.k0 - 1, bad = 0, stopped = 0;
+  if (ld_agent(ctl + kCtlGo) == st.k0) {
+    for (int k = st.k0;; k++) {
       // every flux workgroup's slot holds k; one lane per workgroup polls
       int ok = 1;
       double rmax = 0.0;
@@ -639,7 +640,11 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
       if (bad) break;
       sweep(d, st, sh);
       __syncthreads();
-      const bool more = res > st.tol && k < st.maxit;
+      bool more = res > st.tol && k < st.maxit;
+      if (more && k >= st.kstop) {  // the launch ends here; a later one continues at k + 1
+        more = false;
+        stopped = 1;
+      }
       if (tid == 0) st_agent(ctl + kCtlGo, more ? k + 1 : -1);
       if (!more) break;
     }
@@ -648,6 +653,7 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
     st_agent(ctl + kCtlGo, -1);  // releases the flux workgroups on every exit path
     st.iters[0] = it;
     st.iters[1] = bad;
+    st.iters[2] = stopped;
   }
 }
 
@@ -673,6 +679,14 @@ __global__ void k_ns_feedback(Dev d, SyntheticState st) {
   for (int k = 0; k < 4; k++) {
     d.S[k * N + c] = d.S[k * N + c] + s0[k];
     d.S0[k * N + c] = s0[k] + st.s00[k * N + c];
+  }
+}
+
+// the next launch of the inner loop continues at iteration k0
+__global__ void k_ns_resume(SyntheticState st, int k0) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st.ctl[kCtlGo] = k0;
+    st.iters[2] = 0;
   }
 }
 
@@ -741,16 +755,27 @@ int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st) {
   return 0;
 }
 
-int synthetic_step(const Dev &d, SyntheticState &st, hipStream_t s) {
+int synthetic_begin(const Dev &d, SyntheticState &st, hipStream_t s) {
   if (!st.on) {
     g_syn_err = "synthetic solve not initialised";
     return GSIS_EARG;
   }
-  const int nc = (d.nx - 1) * (d.ny - 1), nci = (d.nx - 5) * (d.ny - 5);
+  const int nc = (d.nx - 1) * (d.ny - 1);
   hipLaunchKernelGGL(k_ns_prepare, dim3((std::max(nc, st.nflux) + 255) / 256), dim3(256), 0, s, d, st);
+  return 0;
+}
+
+int synthetic_run(const Dev &d, SyntheticState &st, int k0, int kstop, hipStream_t s) {
+  if (!st.on || k0 < 1 || kstop < k0) {
+    g_syn_err = "synthetic solve: not initialised, or an empty range of inner iterations";
+    return GSIS_EARG;
+  }
+  if (k0 > 1) hipLaunchKernelGGL(k_ns_resume, dim3(1), dim3(64), 0, s, st, k0);
   if (st.nflux > 0) {
     Dev dd = d;
     SyntheticState ss = st;
+    ss.k0 = k0;
+    ss.kstop = kstop;
     void *args[] = {&dd, &ss};
     const void *fn = st.block <= 256 ? reinterpret_cast<const void *>(k_ns_loop<256>)
                                       : reinterpret_cast<const void *>(k_ns_loop<1024>);
@@ -760,10 +785,33 @@ int synthetic_step(const Dev &d, SyntheticState &st, hipStream_t s) {
       g_syn_err = std::string("synthetic solve: cooperative launch failed: ") + hipGetErrorString(e);
       return GSIS_EHIP;
     }
-  } else {
+  } else if (k0 == 1) {
     hipLaunchKernelGGL(k_ns_empty, dim3(1), dim3(64), 0, s, st);
   }
+  return 0;
+}
+
+void synthetic_finish(const Dev &d, SyntheticState &st, hipStream_t s) {
+  const int nci = (d.nx - 5) * (d.ny - 5);
   if (nci > 0) hipLaunchKernelGGL(k_ns_feedback, dim3((nci + 255) / 256), dim3(256), 0, s, d, st);
+}
+
+int synthetic_step(const Dev &d, SyntheticState &st, hipStream_t s) {
+  int rc = synthetic_begin(d, st, s);
+  if (!rc) rc = synthetic_run(d, st, 1, 0x7fffffff, s);
+  if (!rc) synthetic_finish(d, st, s);
+  return rc;
+}
+
+int synthetic_unfinished(SyntheticState &st, hipStream_t s, bool *unfinished, int *iters) {
+  int h[3] = {0, 0, 0};
+  if (hipMemcpyAsync(h, st.iters, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    g_syn_err = "synthetic solve: reading the loop state failed";
+    return GSIS_EHIP;
+  }
+  *unfinished = h[2] != 0 && h[1] == 0;
+  *iters = h[0];
   return 0;
 }
 

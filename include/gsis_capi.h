@@ -215,6 +215,68 @@ int gsis_destroy(gsis_handle *h);
 
 const char *gsis_last_error(void);
 
+/* ---- Learned closure of the high-order terms (HoT) ------------------------
+ * No counterpart in the reference.  Convert_NS computes the HoT that the
+ * synthetic NS equations add to their stress and heat flux from the DVM
+ * moments (src/Hybrid_Sub.f90:317-349, :419-448); this network predicts the
+ * five that the synthetic solve uses, HoT tau_11, tau_22, tau_12 and q_x, q_y
+ * on an NS face, from 8 local inputs: the strain rates du/dx, dv/dy,
+ * du/dy + dv/dx and the temperature gradient dT/dx, dT/dy of the face (the
+ * gradients of Convert_NS), density and temperature averaged over the two
+ * cells, and log10 Kn.  4 hidden layers of 128 SiLU units; BF16 weights and
+ * activations, FP32 accumulation (MFMA), FP32 master weights and Adam.
+ * Inputs and outputs are standardised with per-feature mean and standard
+ * deviation (gsis_mlp_set_scaling).  Arrays are row-major, one sample per row. */
+#define GSIS_MLP_NIN 8
+#define GSIS_MLP_NOUT 5
+#define GSIS_MLP_WIDTH 128
+#define GSIS_MLP_HIDDEN 4
+typedef struct gsis_mlp gsis_mlp;
+
+/* A network with all parameters zero, on the current device. */
+int gsis_mlp_create(gsis_mlp **out);
+/* Number of parameters: W1 (128x8), b1, W2, b2, W3, b3, W4 (128x128 each),
+ * b4, W5 (5x128), b5 in this order, each W row-major (output by input). */
+int gsis_mlp_nparams(void);
+int gsis_mlp_set_params(gsis_mlp *m, const float *params);
+int gsis_mlp_get_params(gsis_mlp *m, float *params);
+/* scaling[0:8] input means, [8:16] input standard deviations, [16:21] output
+ * means, [21:26] output standard deviations */
+int gsis_mlp_set_scaling(gsis_mlp *m, const float *scaling);
+/* y (n x 5) = network(x (n x 8)), in the features' and HoT's own units. */
+int gsis_mlp_forward(gsis_mlp *m, const float *x, int32_t n, float *y);
+/* Mean squared error of the standardised outputs against y (n x 5) and its
+ * gradient with respect to the parameters (nparams floats), n <= 8192. */
+int gsis_mlp_loss_grad(gsis_mlp *m, const float *x, const float *y, int32_t n, double *loss, float *grad);
+/* Adam on the mean squared error of the standardised outputs: epochs over
+ * the n samples in shuffled batches of `batch` (<= 8192, a multiple of 64),
+ * learning rate lr; loss_per_epoch (epochs doubles, may be NULL) receives each
+ * epoch's mean batch loss. */
+int gsis_mlp_train(gsis_mlp *m, const float *x, const float *y, int32_t n, int32_t epochs, int32_t batch, float lr,
+                   uint64_t seed, double *loss_per_epoch);
+int gsis_mlp_destroy(gsis_mlp *m);
+
+/* Closure faces of a handle's mesh: the NS faces whose gradient stencil holds
+ * only physical cells, i-faces i = 2..nx-1 on rows j = 2..ny-2 (row-major, i
+ * fastest), then j-faces i = 2..nx-2, j = 2..ny-1.  Their 8 network inputs,
+ * computed from the macroscopic state of the last Convert_NS, and the 5 HoT
+ * of that Convert_NS (the network's targets): x (cap x 8), y (cap x 5); *n
+ * receives the number of closure faces; x and y may be NULL to query it.
+ * Fails with GSIS_EARG when cap < *n. */
+int gsis_closure_samples(gsis_handle *h, float *x, float *y, int64_t cap, int64_t *n);
+/* Use the network in the synthetic NS solve (mode M1) of later steps:
+ * every `every` inner iterations the HoT of each accepted closure face is set
+ * to HoT_DVM + N(state now) - N(state of Convert_NS), N the network, so the
+ * synthetic equations follow how the HoT change with the macroscopic state;
+ * the fixed point is the one without the network.  A face is accepted when
+ * the network reproduces its DVM HoT at the step start within `gate`
+ * standard deviations of the training targets in every output; the others
+ * keep the DVM HoT (the fallback).  net NULL switches the closure off.  The
+ * network must outlive its use and is not used concurrently by two handles. */
+int gsis_set_closure(gsis_handle *h, gsis_mlp *net, double gate, int32_t every);
+/* Closure faces, accepted faces and HoT refreshes of the last step. */
+int gsis_closure_stats(gsis_handle *h, int64_t *faces, int64_t *accepted, int32_t *refreshes);
+
 #ifdef __cplusplus
 }
 #endif
