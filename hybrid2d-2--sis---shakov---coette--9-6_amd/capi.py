@@ -71,6 +71,11 @@ EXPORTED = (
     "gsis_mlp_loss_grad",
     "gsis_mlp_train",
     "gsis_mlp_destroy",
+    "gsis_fill_frd_uniform",
+    "gsis_batch_create",
+    "gsis_batch_step",
+    "gsis_batch_tune",
+    "gsis_batch_destroy",
 )
 
 MLP_NIN = 8
@@ -184,7 +189,7 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     c_i64p = ctypes.POINTER(ctypes.c_int64)
     lib.gsis_closure_samples.argtypes = [h, c_fp, c_fp, ctypes.c_int64, c_i64p]
     lib.gsis_set_closure.argtypes = [h, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]
-    lib.gsis_closure_stats.argtypes = [h, c_i64p, c_i64p, ctypes.POINTER(ctypes.c_int32)]
+    lib.gsis_closure_stats.argtypes = [h, c_i64p, c_i64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     lib.gsis_mlp_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
     lib.gsis_mlp_nparams.argtypes = []
     lib.gsis_mlp_set_params.argtypes = [ctypes.c_void_p, c_fp]
@@ -195,6 +200,12 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.gsis_mlp_train.argtypes = [ctypes.c_void_p, c_fp, c_fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                    ctypes.c_float, ctypes.c_uint64, c_dp]
     lib.gsis_mlp_destroy.argtypes = [ctypes.c_void_p]
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    lib.gsis_fill_frd_uniform.argtypes = [h, c_dp, c_dp]
+    lib.gsis_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+    lib.gsis_batch_step.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    lib.gsis_batch_tune.argtypes = [ctypes.c_void_p, ctypes.c_int32, i32p, i32p, c_dp]
+    lib.gsis_batch_destroy.argtypes = [ctypes.c_void_p]
     for name in EXPORTED:
         if name != "gsis_last_error":
             getattr(lib, name).restype = ctypes.c_int

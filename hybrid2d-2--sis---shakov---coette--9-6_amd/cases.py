@@ -728,11 +728,13 @@ def frd_shape(case: Case):
     return (2, case.nx + 7, case.ny + 7, case.nvi, case.nvj)
 
 
-def initial_state(case: Case):
+def initial_state(case: Case, with_frd: bool = True):
     """init_flow_zero (src/sub_init.f90:400-457) + the Maxwellian fill of
     GKUA_sudufenbu (src/GKUA_sudufenbu.f90:22-46).  Returns Fortran-layout
     arrays (frd, s, cf, u).  ``case.init == "rest"`` starts from the gas at
-    rest (n=T=1, u=0)."""
+    rest (n=T=1, u=0).  The initial state is uniform, so with
+    ``with_frd=False`` the first item is instead the pair (g, h) of
+    per-velocity values that every cell of FRD holds."""
     c = case.const
     nx, ny = case.nx, case.ny
     g, ma, aoa = c["gamma"], c["ma"], c["aoa"]
@@ -765,6 +767,9 @@ def initial_state(case: Case):
     vjv = vj - vk
     vij2 = (viu[:, None] ** 2 + vjv[None, :] ** 2) / tk
     gdv = crt * np.exp(-vij2)  # (nvi, nvj)
+    if not with_frd:
+        # the per-velocity values every cell -1..n+1 holds (velocity iv fastest)
+        return (gdv.ravel(order="F").copy(), (tk * gdv / 2.0).ravel(order="F").copy()), s, cf, u
     frd = np.zeros(frd_shape(case), order="F")
     frd[0, 2:nx + 5, 2:ny + 5, :, :] = gdv[None, None, :, :]
     frd[1, 2:nx + 5, 2:ny + 5, :, :] = (tk * gdv / 2.0)[None, None, :, :]

@@ -148,9 +148,10 @@ def set_closure(solver, net: Mlp | None, gate: float = 1.0, every: int = 50) -> 
 
 def stats(solver) -> dict:
     lib = solver.lib
-    f, a, r = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
-    capi.check(lib, lib.gsis_closure_stats(solver.h, ctypes.byref(f), ctypes.byref(a), ctypes.byref(r)))
-    return {"faces": f.value, "accepted": a.value, "refreshes": r.value}
+    f, a, r, x = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+    capi.check(lib, lib.gsis_closure_stats(solver.h, ctypes.byref(f), ctypes.byref(a), ctypes.byref(r),
+                                           ctypes.byref(x)))
+    return {"faces": f.value, "accepted": a.value, "refreshes": r.value, "rejected": x.value}
 
 
 def save(path, params, scaling, meta: dict) -> None:

@@ -224,21 +224,37 @@ void kinetic(gsis_handle *h) {
 
 // The synthetic solve with the learned closure: the inner loop in launches of
 // cl_every iterations, the HoT of the accepted closure faces updated from the
-// network between them; the same iterations as one launch otherwise.
+// network between them.  The residual check: when an updated launch ends
+// with a T<0.1 stop or with max |Res_4| grown by more than kClosureGrow over
+// the launch before, the step falls back to the DVM HoT and continues the
+// loop from its starting state without the network.
+constexpr double kClosureGrow = 1.5;
+
 int synthetic_with_closure(gsis_handle *h) {
   gsis::MlpDev &net = h->net->d;
   int rc = gsis::synthetic_begin(h->d, h->syn, h->s);
   if (rc) return fail(rc, gsis::synthetic_error());
   if (gsis::closure_begin(h->d, h->cl, net, h->cl_gate, h->s)) return fail(GSIS_EHIP, "closure: start failed");
+  bool on = true;
+  double res_prev = 0.0;
   for (int k0 = 1;;) {
-    rc = gsis::synthetic_run(h->d, h->syn, k0, k0 + h->cl_every - 1, h->s);
+    rc = gsis::synthetic_run(h->d, h->syn, k0, on ? k0 + h->cl_every - 1 : 0x7fffffff, h->s);
     if (rc) return fail(rc, gsis::synthetic_error());
-    bool more = false;
+    bool more = false, bad = false;
     int it = 0;
-    rc = gsis::synthetic_unfinished(h->syn, h->s, &more, &it);
+    double res = 0.0;
+    rc = gsis::synthetic_state(h->syn, h->s, &more, &it, &bad, &res);
     if (rc) return fail(rc, gsis::synthetic_error());
+    if (on && h->cl.refreshes > 0 && (bad || !(res <= kClosureGrow * res_prev))) {
+      if (gsis::closure_reject(h->d, h->cl, h->s)) return fail(GSIS_EHIP, "closure: fallback failed");
+      on = false;
+      if (it >= h->syn.maxit) break;
+      k0 = it + 1;
+      continue;
+    }
     if (!more) break;
-    if (gsis::closure_refresh(h->d, h->cl, net, h->s)) return fail(GSIS_EHIP, "closure: HoT update failed");
+    if (on && gsis::closure_refresh(h->d, h->cl, net, h->s)) return fail(GSIS_EHIP, "closure: HoT update failed");
+    res_prev = res;
     k0 = it + 1;
   }
   if (gsis::closure_end(h->d, h->cl, h->s)) return fail(GSIS_EHIP, "closure: restoring the DVM HoT failed");
@@ -355,6 +371,152 @@ int scratch_on(int dev, double **out) {
     g_scratch_ptr[dev] = static_cast<double *>(ptr);
   }
   *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+// ---- batches of instances (gsis_batch_*) ----------------------------------
+struct gsis_batch {
+  std::vector<gsis_handle *> hs;
+  int prec = 0, scheme = 0;
+  int nw = 8, order = 0;  // strip width and job order of the captured step
+  hipStream_t s = nullptr;
+  void *djobs = nullptr;  // device table of StripJob<T>
+  int *epoch = nullptr;   // [0] launch epoch, [1] a seam wait timed out
+  std::vector<int *> flags;
+  int nstrip = 0, nseam = 0, ncellb = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+void batch_drop_graph(gsis_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// job order: as given (0) or the largest kinetic work first (1), so that a
+// persistent launch starts its longest pipelines first
+std::vector<int> batch_order(const gsis_batch *b, int order) {
+  std::vector<int> ix(b->hs.size());
+  for (size_t k = 0; k < ix.size(); k++) ix[k] = (int)k;
+  if (order == 1) {
+    auto work = [&](int k) { const Dev &d = b->hs[k]->d; return (double)d.nx * d.ny * d.nv; };
+    std::stable_sort(ix.begin(), ix.end(), [&](int x, int y) { return work(x) > work(y); });
+  }
+  return ix;
+}
+
+template <typename T>
+int batch_jobs(gsis_batch *b) {
+  std::vector<gsis::StripJob<T>> jobs;
+  int a = 0, c = 0, e = 0;
+  for (int k : batch_order(b, b->order)) {
+    gsis_handle *h = b->hs[k];
+    gsis::StripJob<T> j{};
+    j.d = h->d;
+    j.cp = reinterpret_cast<const T *>(h->cpar);
+    j.F = (T *)h->FG;
+    j.R = (T *)h->RG;
+    j.slot = (const T *)h->slot;
+    j.msg = (T *)h->strip.msg;
+    j.dright = (T *)h->strip.dright;
+    j.flag = b->flags[k];
+    j.oflags = h->strip.oflags;
+    j.ntiles = (h->d.nv + 63) / 64;
+    j.strip0 = a;
+    j.seam0 = c;
+    j.cell0 = e;
+    a += gsis::strip_blocks(h->d, b->nw);
+    c += gsis::seam_blocks(h->d, b->nw);
+    e += gsis::cell_blocks(h->d);
+    jobs.push_back(j);
+  }
+  b->nstrip = a;
+  b->nseam = c;
+  b->ncellb = e;
+  HIPCHK(hipMemcpy(b->djobs, jobs.data(), jobs.size() * sizeof(jobs[0]), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// the M0 step of every instance as one graph: per instance S0 <- S and the
+// kinetic BCs, the batched kinetic step, per instance moments + DETFPwall,
+// Convert_NS and the HoT
+template <typename T>
+int batch_capture(gsis_batch *b) {
+  batch_drop_graph(b);
+  int rc = batch_jobs<T>(b);
+  if (rc) return rc;
+  std::vector<hipStream_t> own(b->hs.size());
+  for (size_t k = 0; k < b->hs.size(); k++) {
+    own[k] = b->hs[k]->s;
+    b->hs[k]->s = b->s;
+  }
+  hipError_t e = hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (gsis_handle *h : b->hs) {
+      gsis::launch_save_s0(h->d, b->s);
+      phy_boundary<T>(h);
+    }
+    gsis::launch_strip_batch<T>((const gsis::StripJob<T> *)b->djobs, (int)b->hs.size(), b->nw, b->scheme, b->nstrip,
+                                b->nseam, b->ncellb, b->epoch, b->epoch + 1, b->s);
+    for (gsis_handle *h : b->hs) {
+      moments<T>(h);
+      gsis::launch_convert(h->d, b->s);
+      if (h->p.if_viscous) gsis::launch_hot(h->d, b->s);
+    }
+    const hipError_t e2 = hipStreamEndCapture(b->s, &b->graph);
+    e = e2 != hipSuccess ? e2 : hipGetLastError();
+  }
+  for (size_t k = 0; k < b->hs.size(); k++) b->hs[k]->s = own[k];
+  if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    batch_drop_graph(b);
+    return fail(GSIS_EHIP, std::string("batch: graph capture failed: ") + hipGetErrorString(e));
+  }
+  return 0;
+}
+
+int batch_capture_any(gsis_batch *b) {
+  return b->prec == GSIS_PREC_FP32 ? batch_capture<float>(b) : batch_capture<double>(b);
+}
+
+int batch_replay(gsis_batch *b, int nsteps) {
+  bool m1 = false;
+  for (gsis_handle *h : b->hs) {
+    HIPCHK(hipStreamSynchronize(h->s));
+    m1 = m1 || h->p.mode == GSIS_MODE_M1;
+  }
+  for (int k = 0; k < nsteps; k++) {
+    HIPCHK(hipGraphLaunch(b->exec, b->s));
+    if (!m1) continue;
+    // the synthetic solve of each M1 instance after the captured part of its
+    // step (a cooperative launch and host-side loop control: not captured)
+    HIPCHK(hipStreamSynchronize(b->s));
+    for (gsis_handle *h : b->hs) {
+      if (h->p.mode != GSIS_MODE_M1) continue;
+      int rc = h->net ? synthetic_with_closure(h) : gsis::synthetic_step(h->d, h->syn, h->s);
+      if (rc) return h->net ? rc : fail(rc, gsis::synthetic_error());
+    }
+    for (gsis_handle *h : b->hs) HIPCHK(hipStreamSynchronize(h->s));
+  }
+  HIPCHK(hipStreamSynchronize(b->s));
+  for (gsis_handle *h : b->hs) {
+    h->tt += nsteps * h->p.dt;
+    h->kstep += nsteps;
+  }
+  int w[2] = {0, 0};
+  HIPCHK(hipMemcpy(w, b->epoch, sizeof(w), hipMemcpyDeviceToHost));
+  if (w[1]) return fail(GSIS_EHIP, "batch: a wait for a neighbouring strip timed out");
+  for (gsis_handle *h : b->hs) {
+    const int rc = check_err_flag(h);
+    if (rc) return rc;
+  }
   return 0;
 }
 
@@ -592,7 +754,7 @@ int gsis_set_closure(gsis_handle *h, gsis_mlp *net, double gate, int32_t every) 
   return 0;
 }
 
-int gsis_closure_stats(gsis_handle *h, int64_t *faces, int64_t *accepted, int32_t *refreshes) {
+int gsis_closure_stats(gsis_handle *h, int64_t *faces, int64_t *accepted, int32_t *refreshes, int32_t *rejected) {
   if (!h) return fail(GSIS_EARG, "NULL handle");
   int acc = 0;
   if (h->cl.count) {
@@ -602,6 +764,7 @@ int gsis_closure_stats(gsis_handle *h, int64_t *faces, int64_t *accepted, int32_
   if (faces) *faces = h->cl.mem ? h->cl.nf : gsis::closure_faces(h->d, nullptr);
   if (accepted) *accepted = acc;
   if (refreshes) *refreshes = h->cl.refreshes;
+  if (rejected) *rejected = h->cl.rejected;
   return 0;
 }
 
@@ -815,6 +978,122 @@ int gsis_set_counters(gsis_handle *h, int32_t kstep, double tt) {
   // the counters are read when a step is queued; no wait on the stream
   h->kstep = kstep;
   h->tt = tt;
+  return 0;
+}
+
+int gsis_fill_frd_uniform(gsis_handle *h, const double *g, const double *hv) {
+  if (!h || !g || !hv) return fail(GSIS_EARG, "NULL argument");
+  const size_t nv = h->d.nv;
+  double *dv = nullptr;
+  HIPCHK(hipMalloc(&dv, 2 * nv * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(dv, g, nv * 8, hipMemcpyHostToDevice, h->s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dv + nv, hv, nv * 8, hipMemcpyHostToDevice, h->s);
+  if (e == hipSuccess) {
+    if (h->prec == GSIS_PREC_FP32)
+      gsis::launch_fill_uniform<float>(h->d, dv, dv + nv, (float *)h->FG, (float *)h->FH, (float *)h->slot, h->s);
+    else
+      gsis::launch_fill_uniform<double>(h->d, dv, dv + nv, (double *)h->FG, (double *)h->FH, (double *)h->slot, h->s);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->s);
+  (void)hipFree(dv);
+  if (e != hipSuccess) return fail(GSIS_EHIP, std::string("fill_frd_uniform: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int gsis_batch_create(gsis_handle *const *hs, int32_t n, gsis_batch **out) {
+  if (!out || !hs || n < 1) return fail(GSIS_EARG, "batch: NULL argument or no instance");
+  *out = nullptr;
+  for (int k = 0; k < n; k++) {
+    const gsis_handle *h = hs[k];
+    if (!h) return fail(GSIS_EARG, "batch: NULL handle");
+    if (h->prec != hs[0]->prec || h->d.scheme != hs[0]->d.scheme || h->dev != hs[0]->dev)
+      return fail(GSIS_EARG, "batch: instances must share precision, flux scheme and device");
+    if (h->d.nx - 1 < 2 || h->d.ny - 1 < 2 || gsis::strip_rows(h->d, 8) * 256 > h->strip.msg_elems)
+      return fail(GSIS_EUNSUPPORTED, "batch: instance " + std::to_string(k) + " does not fit the column-wavefront kernel");
+    for (int m = 0; m < k; m++)
+      if (hs[m] == h) return fail(GSIS_EARG, "batch: a handle is listed twice");
+  }
+  gsis_batch *b = new gsis_batch();
+  b->hs.assign(hs, hs + n);
+  b->prec = hs[0]->prec;
+  b->scheme = hs[0]->d.scheme;
+  auto bail = [&](int rc) {
+    gsis_batch_destroy(b);
+    return rc;
+  };
+  auto alloc = [&](void **p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess || hipMemset(*p, 0, bytes) != hipSuccess) return false;
+    b->allocs.push_back(*p);
+    return true;
+  };
+  if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(GSIS_EHIP, "batch: hipStreamCreate failed"));
+  const size_t jb = (size_t)n * std::max(sizeof(gsis::StripJob<double>), sizeof(gsis::StripJob<float>));
+  if (!alloc(&b->djobs, jb) || !alloc((void **)&b->epoch, 64)) return bail(fail(GSIS_EHIP, "batch: allocation failed"));
+  for (int k = 0; k < n; k++) {
+    void *f = nullptr;
+    if (!alloc(&f, std::max<size_t>(gsis::strip_rows(hs[k]->d, 8), 1) * sizeof(int)))
+      return bail(fail(GSIS_EHIP, "batch: allocation failed"));
+    b->flags.push_back((int *)f);
+  }
+  const int rc = batch_capture_any(b);
+  if (rc) return bail(rc);
+  *out = b;
+  return 0;
+}
+
+int gsis_batch_step(gsis_batch *b, int32_t nsteps) {
+  if (!b || nsteps < 0) return fail(GSIS_EARG, "batch: bad argument");
+  return batch_replay(b, nsteps);
+}
+
+int gsis_batch_tune(gsis_batch *b, int32_t steps_per_trial, int32_t *nw, int32_t *order, double *ms_per_step) {
+  if (!b || steps_per_trial < 1) return fail(GSIS_EARG, "batch: bad argument");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best = 1e300;
+  int bnw = b->nw, bord = b->order;
+  int rc = 0;
+  for (int cw : {8, 16}) {
+    for (int co : {0, 1}) {
+      b->nw = cw;
+      b->order = co;
+      if ((rc = batch_capture_any(b))) break;
+      (void)hipEventRecord(e0, b->s);
+      if ((rc = batch_replay(b, steps_per_trial))) break;
+      (void)hipEventRecord(e1, b->s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms / steps_per_trial < best) {
+        best = ms / steps_per_trial;
+        bnw = cw;
+        bord = co;
+      }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  b->nw = bnw;
+  b->order = bord;
+  if ((rc = batch_capture_any(b))) return rc;
+  if (nw) *nw = bnw;
+  if (order) *order = bord;
+  if (ms_per_step) *ms_per_step = best;
+  return 0;
+}
+
+int gsis_batch_destroy(gsis_batch *b) {
+  if (!b) return 0;
+  if (b->s) (void)hipStreamSynchronize(b->s);
+  batch_drop_graph(b);
+  for (void *q : b->allocs) (void)hipFree(q);
+  if (b->s) (void)hipStreamDestroy(b->s);
+  delete b;
   return 0;
 }
 

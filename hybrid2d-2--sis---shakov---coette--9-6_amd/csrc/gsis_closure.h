@@ -17,7 +17,9 @@ struct ClosureBufs {
   float *t = nullptr;        // [nf][5] DVM HoT of Convert_NS
   int *accept = nullptr;     // [nf] 1 where the closure is used
   int *count = nullptr;      // accepted faces of the last step
+  double *usave = nullptr;   // [4][ncell]: U at the start of the inner loop
   int refreshes = 0;         // HoT updates of the last step
+  int rejected = 0;          // the last step fell back to the DVM HoT
   void *mem = nullptr;
   size_t bytes = 0;
 };
@@ -32,12 +34,15 @@ void closure_free(ClosureBufs &b);
 void launch_cl_features(const Dev &d, int nfi, int nf, bool from_prim, float *x, hipStream_t s);
 // the five DVM HoT of the closure faces
 void launch_cl_targets(const Dev &d, int nfi, int nf, float *y, hipStream_t s);
-// start of a synthetic solve: keep the DVM HoT, evaluate the network at the
-// Convert_NS state, decide which faces use the closure
+// start of a synthetic solve: keep the DVM HoT and the starting state U,
+// evaluate the network at the Convert_NS state, decide which faces use the
+// closure
 int closure_begin(const Dev &d, ClosureBufs &b, MlpDev &net, float gate, hipStream_t s);
 // between inner-loop chunks: HoT of the accepted faces from the current state
 int closure_refresh(const Dev &d, ClosureBufs &b, MlpDev &net, hipStream_t s);
 // end of the synthetic solve: the DVM HoT back in place
 int closure_end(const Dev &d, ClosureBufs &b, hipStream_t s);
+// the closure rejected: the DVM HoT and the inner loop's starting state U back
+int closure_reject(const Dev &d, ClosureBufs &b, hipStream_t s);
 
 }  // namespace gsis

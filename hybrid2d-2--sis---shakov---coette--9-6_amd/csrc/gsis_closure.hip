@@ -173,13 +173,14 @@ int closure_alloc(const Dev &d, ClosureBufs &b) {
   if (b.mem) return 0;
   b.nf = closure_faces(d, &b.nfi);
   const size_t nf = std::max(b.nf, 1), N = d.ncell;
-  b.bytes = 12 * N * 8 + nf * (kMlpIn + 3 * kMlpOut) * 4 + nf * 4 + 64;
+  b.bytes = 16 * N * 8 + nf * (kMlpIn + 3 * kMlpOut) * 4 + nf * 4 + 64;
   if (hipMalloc(&b.mem, b.bytes) != hipSuccess) {
     b.mem = nullptr;
     return GSIS_EHIP;
   }
   char *p = reinterpret_cast<char *>(b.mem);
   b.saved = reinterpret_cast<double *>(p); p += 12 * N * 8;
+  b.usave = reinterpret_cast<double *>(p); p += 4 * N * 8;
   b.x = reinterpret_cast<float *>(p); p += nf * kMlpIn * 4;
   b.ybase = reinterpret_cast<float *>(p); p += nf * kMlpOut * 4;
   b.y = reinterpret_cast<float *>(p); p += nf * kMlpOut * 4;
@@ -206,8 +207,11 @@ void launch_cl_targets(const Dev &d, int nfi, int nf, float *y, hipStream_t s) {
 }
 
 int closure_begin(const Dev &d, ClosureBufs &b, MlpDev &net, float gate, hipStream_t s) {
-  if (copy_hot(d, b.saved, true, s) != hipSuccess) return GSIS_EHIP;
+  if (copy_hot(d, b.saved, true, s) != hipSuccess ||
+      hipMemcpyAsync(b.usave, d.U, 4 * (size_t)d.ncell * 8, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return GSIS_EHIP;
   b.refreshes = 0;
+  b.rejected = 0;
   if (b.nf <= 0) return 0;
   launch_cl_features(d, b.nfi, b.nf, true, b.x, s);
   mlp_forward_dev(net, b.x, b.nf, b.ybase, s);
@@ -230,6 +234,14 @@ int closure_refresh(const Dev &d, ClosureBufs &b, MlpDev &net, hipStream_t s) {
 
 int closure_end(const Dev &d, ClosureBufs &b, hipStream_t s) {
   return copy_hot(d, b.saved, false, s) == hipSuccess ? 0 : GSIS_EHIP;
+}
+
+int closure_reject(const Dev &d, ClosureBufs &b, hipStream_t s) {
+  b.rejected = 1;
+  if (copy_hot(d, b.saved, false, s) != hipSuccess ||
+      hipMemcpyAsync(d.U, b.usave, 4 * (size_t)d.ncell * 8, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return GSIS_EHIP;
+  return 0;
 }
 
 }  // namespace gsis

@@ -50,6 +50,34 @@ size_t strip_rows(const Dev &d, int nw);
 // the seam update; false when the mesh or the buffers do not fit.
 template <typename T> bool launch_strip(const Dev &d, double *cpar, T *F, const T *slot, T *R, int store_rhs, int nw, StripBufs &b, hipStream_t s);
 template <typename T> void launch_res4(const Dev &d, const T *RG, const T *RH, hipStream_t s);
+// One instance of a batched kinetic step: its device state and the first
+// workgroup of its share of each batched launch.  The seam flags are the
+// batch's own (their values are the batch's launch epochs).
+template <typename T>
+struct StripJob {
+  Dev d;
+  const T *cp;  // cell constants in T (k_cellpar), by cell
+  T *F, *R;
+  const T *slot;
+  T *msg, *dright;
+  int *flag;
+  const int *oflags;
+  int ntiles;
+  int strip0, seam0, cell0;  // first workgroup in the strip, seam and cell-constant launches
+};
+// workgroups of one job in the batched launches with nw columns per strip workgroup
+int strip_blocks(const Dev &d, int nw);
+int seam_blocks(const Dev &d, int nw);
+int cell_blocks(const Dev &d);
+// the kinetic step of every job (djobs: device table, host copies of the
+// counts): epoch += 1, cell constants, one column-wavefront launch over all
+// jobs' strips (no RHS kept), the seam update; scheme and T shared by all jobs
+template <typename T>
+void launch_strip_batch(const StripJob<T> *djobs, int njobs, int nw, int scheme, int nstrip, int nseam, int ncellb,
+                        int *epoch, int *werr, hipStream_t s);
+// every cell's distribution set to g[v] (G) and h[v] (H), wall slots zero
+template <typename T>
+void launch_fill_uniform(const Dev &d, const double *g, const double *h, T *FG, T *FH, T *slot, hipStream_t s);
 // FRD (Fortran layout) to and from the device layout for velocities v0..v0+nvc-1 through a stage holding that
 // contiguous slice of the Fortran FRD array (2(nx+7)(ny+7) doubles per velocity)
 template <typename T>

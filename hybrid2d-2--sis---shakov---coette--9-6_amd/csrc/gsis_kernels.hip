@@ -627,8 +627,7 @@ constexpr int kNW = kNCellPar;
 // With pstride = 0 the constants are stored by cell instead, cpar[((j-1)*(nx-1) + i-1)*kNW + k]
 // (the layout of the column-wavefront kernel).
 template <typename P>
-__global__ void k_cellpar(Dev d, P *cpar, int pstride) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cellpar_body(const Dev &d, P *cpar, int pstride, int t) {
   const int ncol = d.nx - 1, nrow = d.ny - 1;
   if (t >= ncol * nrow) return;
   const int i = t % ncol + 1, j = t / ncol + 1;
@@ -644,6 +643,10 @@ __global__ void k_cellpar(Dev d, P *cpar, int pstride) {
                             d.nj1[g0] * d.sj[g0], d.nj2[g0] * d.sj[g0], d.nj1[gj] * d.sj[gj], d.nj2[gj] * d.sj[gj]};
 #pragma unroll
   for (int k = 0; k < kNW; k++) o[k] = P(vals[k]);
+}
+template <typename P>
+__global__ void k_cellpar(Dev d, P *cpar, int pstride) {
+  cellpar_body(d, cpar, pstride, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // Flux through one face in the fused kernel: the rules of iface_flux /
@@ -1130,20 +1133,32 @@ __device__ __forceinline__ void lds_copy16(const void *src, void *dst) {
 // SL: slope rule of the fluxes (0 minmod for NND, 1 WENO3, 2 none: UP1).  At least
 // 4 wavefronts per SIMD (at most 128 VGPRs): with 80 KB of LDS per 8-column
 // workgroup two workgroups share a CU, which the register file must allow.
-template <typename T, int NW, int SL>
-__global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict__ cpar, T *__restrict__ F,
-                                                 const T *__restrict__ slot, T *__restrict__ R, int store_rhs,
-                                                 int ntiles, T *msg, T *dright, int *flag, int epoch, int *werr,
-                                                 const int *__restrict__ oflags) {
-  using T2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
-  constexpr unsigned ES = sizeof(T);
-  __shared__ T2 XG[2][NW][64], XH[2][NW][64];  // (face flux, Delta*) of each column
+template <typename T>
+using Vec2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+
+// LDS of one workgroup of the column-wavefront kernel (declared by each
+// kernel, so that its addresses are constants of that kernel)
+template <typename T, int NW>
+struct StripLds {
+  Vec2<T> XG[2][NW][64], XH[2][NW][64];  // (face flux, Delta*) of each column
   // Values of the next row, copied from global memory straight into LDS
   // (global_load_lds, 16 bytes per lane) one step ahead, so that no vector
   // register holds them while they are in flight: per column, slots
   // 0 F(i, r+2), 1 F(i-1, r), 2 F(i+1, r), 3 F(i+2, r), 4 x-boundary value,
   // 5 y-boundary value, each the tile's 64 velocities of G then of H.
-  __shared__ T STG[NW][6][128];
+  T STG[NW][6][128];
+};
+
+template <typename T, int NW, int SL>
+__device__ __forceinline__ void strip_body(StripLds<T, NW> &lds, const Dev &d, const T *__restrict__ cpar,
+                                           T *__restrict__ F, const T *__restrict__ slot, T *__restrict__ R,
+                                           int store_rhs, int ntiles, T *msg, T *dright, int *flag, int epoch,
+                                           int *werr, const int *__restrict__ oflags, int bid) {
+  using T2 = Vec2<T>;
+  constexpr unsigned ES = sizeof(T);
+  auto &XG = lds.XG;
+  auto &XH = lds.XH;
+  auto &STG = lds.STG;
   constexpr int kPer = 16 / sizeof(T);  // velocities per lane and copy
   constexpr int kLanes = 64 / kPer;     // lanes per component
   const int nx = d.nx, ny = d.ny, ncol = nx - 1, nrow = ny - 1, nv = d.nv;
@@ -1151,7 +1166,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
   const int LT = nrow + 4;                 // positions: virtual rows -3..0, rows 1..ny-1
   const int nsteps = (NW - 1) + LT + 2;
   const int w0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int g0w = (int)blockIdx.x / ntiles, tile = (int)blockIdx.x - g0w * ntiles;
+  const int g0w = bid / ntiles, tile = bid - g0w * ntiles;
   const int i0 = g0w * NW + w0 + 1;  // mesh column of this wavefront
   // Strips early in the pipeline feed all later ones: their wavefronts get
   // the higher issue priority on a CU they share with later strips.
@@ -1169,7 +1184,9 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
     vy = T(d.vj[vv / d.nvi]);
   }
   constexpr T hs = SL == 2 ? T(0) : T(0.5);  // slope factor (0: first-order upwind)
-  const int cfl0 = oflags[ny + 2 + i0];  // j-face overrides of column i
+  // (override words are wavefront-uniform: scalar registers also where the
+  // table's address was itself read from memory, as in the batched kernel)
+  const int cfl0 = __builtin_amdgcn_readfirstlane(oflags[ny + 2 + i0]);  // j-face overrides of column i
   // Roles, re-derived every step: the first column of strip g > 0 reads the
   // seam message of strip g-1 (consumer); the last column of a strip that is
   // not the last writes it (producer) and is updated by k_seam; xlo / xhi:
@@ -1286,7 +1303,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
       pf3 = pc[kRec / 4 - 1];
     }
     const bool real = r >= 1 && r <= nrow;
-    const int rfl = real ? oflags[r] : 0;  // i-face overrides of row r
+    const int rfl = real ? __builtin_amdgcn_readfirstlane(oflags[r]) : 0;  // i-face overrides of row r
     // the values staged in the previous step for this row
     // (slots that were not copied for this row hold older values, which are
     // read here but never used)
@@ -1337,7 +1354,7 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
       copy(w, 1, Fb + oc + rb - cb, stF, lane);
       copy(w, 2, Fb + oc + rb + cb, stF, lane);
       copy(w, 3, Fb + oc + rb + 2 * cb, stF, lane);
-      const int rfn = oflags[rn];
+      const int rfn = __builtin_amdgcn_readfirstlane(oflags[rn]);
       if (xlo) {
         if (rfn & 1) copy(w, 4, slotrow(1, rn + 1), stS, lane);
         else copy(w, 4, Fb + oc + rb - 2 * cb, stF, lane);  // ghost cell (-1, rn)
@@ -1454,15 +1471,83 @@ __global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict
   }
 }
 
+template <typename T, int NW, int SL>
+__global__ void __launch_bounds__(64 * NW, 4) k_strip(Dev d, const T *__restrict__ cpar, T *__restrict__ F,
+                                                 const T *__restrict__ slot, T *__restrict__ R, int store_rhs,
+                                                 int ntiles, T *msg, T *dright, int *flag, int epoch, int *werr,
+                                                 const int *__restrict__ oflags) {
+  __shared__ StripLds<T, NW> lds;
+  strip_body<T, NW, SL>(lds, d, cpar, F, slot, R, store_rhs, ntiles, msg, dright, flag, epoch, werr, oflags,
+                        (int)blockIdx.x);
+}
+
+// wavefront-uniform copies (scalar registers) of values read from memory
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+template <typename P>
+__device__ __forceinline__ P *uni(P *p) {
+  const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v), hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return reinterpret_cast<P *>(((unsigned long long)hi << 32) | lo);
+}
+
+// the job of workgroup b of a batched launch: the last job whose first
+// workgroup (member K of the jobs) is <= b (wavefront-uniform)
+template <typename J, int J::*K>
+__device__ __forceinline__ int job_of(const J *jobs, int njobs, int b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].*K <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// One launch of the column-wavefront kernel over the strips of every job of a
+// batch (instances of different mesh and velocity grid).  Each job's
+// workgroups are consecutive and strip-major, so a workgroup still waits only
+// for lower-numbered workgroups of its own job.  The launch epoch is read
+// from device memory (a replayed graph increments it in k_batch_epoch).
+template <typename T, int NW, int SL>
+__global__ void __launch_bounds__(64 * NW, 4) k_strip_batch(const StripJob<T> *__restrict__ jobs, int njobs,
+                                                       const int *__restrict__ epoch, int *werr) {
+  const int k = job_of<StripJob<T>, &StripJob<T>::strip0>(jobs, njobs, (int)blockIdx.x);
+  const StripJob<T> &J = jobs[k];
+  // the fields the kernel uses, in scalar registers like kernel arguments
+  Dev d{};
+  d.nx = uni(J.d.nx);
+  d.ny = uni(J.d.ny);
+  d.nvi = uni(J.d.nvi);
+  d.nv = uni(J.d.nv);
+  d.NI = uni(J.d.NI);
+  d.ncell = uni(J.d.ncell);
+  d.L = uni(J.d.L);
+  d.vi = uni(J.d.vi);
+  d.vj = uni(J.d.vj);
+  __shared__ StripLds<T, NW> lds;
+  strip_body<T, NW, SL>(lds, d, uni(J.cp), uni(J.F), uni(J.slot), uni(J.R), 0, uni(J.ntiles), uni(J.msg), uni(J.dright),
+                        uni(J.flag), uni(*epoch), werr, uni(J.oflags), (int)blockIdx.x - uni(J.strip0));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cellpar_batch(const StripJob<T> *__restrict__ jobs, int njobs) {
+  const int k = job_of<StripJob<T>, &StripJob<T>::cell0>(jobs, njobs, (int)blockIdx.x);
+  const StripJob<T> &J = jobs[k];
+  cellpar_body(J.d, const_cast<T *>(J.cp), 0, ((int)blockIdx.x - J.cell0) * 256 + (int)threadIdx.x);
+}
+
+__global__ void k_batch_epoch(int *epoch) {
+  if (threadIdx.x == 0) *epoch += 1;
+}
+
 // Backward pass + update of the last column of every strip but the last
 // (src/GKUA_Residual.f90:64-109): Delta*(i, j) and Delta*(i, j+1) from the
 // seam messages, Delta*(i+1, j) from dright, the coefficients
 // -A^-_x/alfa, -A^-_y/alfa recomputed from the cell constants as k_strip
 // computes them.  One thread per (seam, tile, row, velocity lane).
 template <typename T>
-__global__ void __launch_bounds__(256) k_seam(Dev d, const T *__restrict__ cpar, T *F, const T *msg,
-                                              const T *dright, int ntiles, int nw) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void seam_body(const Dev &d, const T *__restrict__ cpar, T *F, const T *msg,
+                                          const T *dright, int ntiles, int nw, size_t t) {
   const int lane = (int)(t & 63);
   const size_t row_id = t >> 6;  // (seam * ntiles + tile) * nrow + r - 1
   const int ncol = d.nx - 1, nrow = d.ny - 1, nv = d.nv;
@@ -1495,6 +1580,18 @@ __global__ void __launch_bounds__(256) k_seam(Dev d, const T *__restrict__ cpar,
   if ((double)fh < kClamp) fh = T(kClamp);
   F[o] = fg;
   F[nf + o] = fh;
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_seam(Dev d, const T *__restrict__ cpar, T *F, const T *msg,
+                                              const T *dright, int ntiles, int nw) {
+  seam_body(d, cpar, F, msg, dright, ntiles, nw, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_seam_batch(const StripJob<T> *__restrict__ jobs, int njobs, int nw) {
+  const int k = job_of<StripJob<T>, &StripJob<T>::seam0>(jobs, njobs, (int)blockIdx.x);
+  const StripJob<T> &J = jobs[k];
+  seam_body(J.d, J.cp, J.F, J.msg, J.dright, J.ntiles, nw, (size_t)((int)blockIdx.x - J.seam0) * 256 + threadIdx.x);
 }
 
 // Conserved moments of the stored RHS per interior cell (GaussLegendre_rs,
@@ -2076,6 +2173,53 @@ bool launch_strip_nw(const Dev &d, double *cpar, T *F, const T *slot, T *R, int 
   }
   return true;
 }
+int strip_blocks(const Dev &d, int nw) {
+  const int ncol = d.nx - 1;
+  return (ncol + nw - 1) / nw * ((d.nv + 63) / 64);
+}
+int seam_blocks(const Dev &d, int nw) { return (int)((strip_rows(d, nw) * 64 + 255) / 256); }
+int cell_blocks(const Dev &d) { return ((d.nx - 1) * (d.ny - 1) + 255) / 256; }
+
+template <typename T, int NW>
+void strip_batch_nw(const StripJob<T> *djobs, int njobs, int scheme, int nstrip, int *epoch, int *werr,
+                    hipStream_t s) {
+  if (scheme == kSchemeWeno3)
+    hipLaunchKernelGGL((k_strip_batch<T, NW, 1>), dim3(nstrip), dim3(64 * NW), 0, s, djobs, njobs, epoch, werr);
+  else if (scheme == 0)
+    hipLaunchKernelGGL((k_strip_batch<T, NW, 2>), dim3(nstrip), dim3(64 * NW), 0, s, djobs, njobs, epoch, werr);
+  else
+    hipLaunchKernelGGL((k_strip_batch<T, NW, 0>), dim3(nstrip), dim3(64 * NW), 0, s, djobs, njobs, epoch, werr);
+}
+
+template <typename T>
+void launch_strip_batch(const StripJob<T> *djobs, int njobs, int nw, int scheme, int nstrip, int nseam, int ncellb,
+                        int *epoch, int *werr, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_epoch, dim3(1), dim3(64), 0, s, epoch);
+  hipLaunchKernelGGL(k_cellpar_batch<T>, dim3(ncellb), dim3(256), 0, s, djobs, njobs);
+  if (nw == 16) strip_batch_nw<T, 16>(djobs, njobs, scheme, nstrip, epoch, werr, s);
+  else strip_batch_nw<T, 8>(djobs, njobs, scheme, nstrip, epoch, werr, s);
+  if (nseam > 0) hipLaunchKernelGGL(k_seam_batch<T>, dim3(nseam), dim3(256), 0, s, djobs, njobs, nw);
+}
+
+template <typename T>
+__global__ void k_fill_uniform(Dev d, const double *g, const double *h, T *FG, T *FH, T *slot, size_t nslot) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nf = (size_t)d.ncell * d.nv;
+  if (t < nf) {
+    const int v = (int)(t % d.nv);
+    FG[t] = T(g[v]);
+    FH[t] = T(h[v]);
+  }
+  if (t < nslot) slot[t] = T(0);
+}
+
+template <typename T>
+void launch_fill_uniform(const Dev &d, const double *g, const double *h, T *FG, T *FH, T *slot, hipStream_t s) {
+  const size_t nf = (size_t)d.ncell * d.nv, ns = (size_t)4 * d.L * 2 * d.nv;
+  const size_t n = std::max(nf, ns);
+  hipLaunchKernelGGL(k_fill_uniform<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, g, h, FG, FH, slot, ns);
+}
+
 size_t strip_rows(const Dev &d, int nw) {
   const int ncol = d.nx - 1, nrow = d.ny - 1;
   if (ncol < 2 || nrow < 2) return 0;
@@ -2170,7 +2314,9 @@ void launch_residual(const Dev &d, double *out, hipStream_t s) {
   template void launch_res4<T>(const Dev &, const T *, const T *, hipStream_t);                  \
   template void launch_frd_out_chunk<T>(const Dev &, double *, int, int, const T *, const T *, const T *,       \
                                         hipStream_t);                                                           \
-  template void launch_frd_in_chunk<T>(const Dev &, const double *, int, int, T *, T *, T *, hipStream_t);
+  template void launch_frd_in_chunk<T>(const Dev &, const double *, int, int, T *, T *, T *, hipStream_t);    \
+  template void launch_strip_batch<T>(const StripJob<T> *, int, int, int, int, int, int, int *, int *, hipStream_t); \
+  template void launch_fill_uniform<T>(const Dev &, const double *, const double *, T *, T *, T *, hipStream_t);
 INST(double)
 INST(float)
 #undef INST

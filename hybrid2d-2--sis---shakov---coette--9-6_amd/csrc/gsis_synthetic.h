@@ -42,7 +42,8 @@ struct SyntheticState {
   double *dump = nullptr;    // [4][64] sink of the sweep's stores for lanes without a cell
   int *ctl = nullptr;        // [1] go (next iteration or -1), [3] a wait timed out
   int *iters = nullptr;      // [0]: inner iterations of the last step, [1]: T<0.1 flag,
-                             // [2]: the launch stopped at kstop with the loop unfinished
+                             // [2]: the launch stopped at kstop with the loop unfinished,
+                             // [4:6]: max |Res_4| of the launch's last iteration (double)
   void *mem = nullptr;       // single allocation holding all of the above
   int k0 = 1;                // first inner iteration of a launch
   int kstop = 0x7fffffff;    // last inner iteration a launch may run
@@ -58,9 +59,10 @@ int synthetic_step(const Dev &d, SyntheticState &st, hipStream_t s);
 int synthetic_begin(const Dev &d, SyntheticState &st, hipStream_t s);
 int synthetic_run(const Dev &d, SyntheticState &st, int k0, int kstop, hipStream_t s);
 void synthetic_finish(const Dev &d, SyntheticState &st, hipStream_t s);
-// after synthetic_run: whether the loop stopped at kstop unfinished, and the
-// inner iterations run so far (synchronises the stream)
-int synthetic_unfinished(SyntheticState &st, hipStream_t s, bool *unfinished, int *iters);
+// after synthetic_run: whether the loop stopped at kstop unfinished, the
+// inner iterations run so far, the T<0.1 stop and max |Res_4| of the last
+// iteration (synchronises the stream)
+int synthetic_state(SyntheticState &st, hipStream_t s, bool *unfinished, int *iters, bool *bad_t, double *res);
 int synthetic_read_iters(SyntheticState &st, hipStream_t s, int *iters, int *bad_t);
 void synthetic_free(SyntheticState &st);
 const char *synthetic_error();

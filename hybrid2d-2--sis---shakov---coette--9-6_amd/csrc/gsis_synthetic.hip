@@ -30,6 +30,7 @@
 // rule: iteration k+1 runs when max |Res_4| of iteration k exceeds ns_tol and
 // k < ns_maxit.  Every wait between workgroups is bounded; a timeout ends the
 // loop and is reported by the host.
+#include <cstring>
 #include <string>
 
 #include "gsis_ns_common.h"
@@ -614,6 +615,7 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
   }
   // workgroup 0: the LU-SGS sweeps and the loop control
   int it = st.k0 - 1, bad = 0, stopped = 0;
+  double res_last = 0.0;
   if (ld_agent(ctl + kCtlGo) == st.k0) {
     for (int k = st.k0;; k++) {
       // every flux workgroup's slot holds k; one lane per workgroup polls
@@ -636,6 +638,7 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
       for (int q = 1; q < (int)(blockDim.x >> 6); q++) res = fmax(res, sh.red[q]);
       bad = b;
       it = k;
+      res_last = res;
       __syncthreads();
       if (bad) break;
       sweep(d, st, sh);
@@ -654,6 +657,7 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
     st.iters[0] = it;
     st.iters[1] = bad;
     st.iters[2] = stopped;
+    reinterpret_cast<double *>(st.iters + 4)[0] = res_last;  // max |Res_4| of the launch's last iteration
   }
 }
 
@@ -686,6 +690,7 @@ __global__ void k_ns_feedback(Dev d, SyntheticState st) {
 __global__ void k_ns_resume(SyntheticState st, int k0) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     st.ctl[kCtlGo] = k0;
+    st.iters[1] = 0;
     st.iters[2] = 0;
   }
 }
@@ -803,8 +808,8 @@ int synthetic_step(const Dev &d, SyntheticState &st, hipStream_t s) {
   return rc;
 }
 
-int synthetic_unfinished(SyntheticState &st, hipStream_t s, bool *unfinished, int *iters) {
-  int h[3] = {0, 0, 0};
+int synthetic_state(SyntheticState &st, hipStream_t s, bool *unfinished, int *iters, bool *bad_t, double *res) {
+  int h[6] = {0, 0, 0, 0, 0, 0};
   if (hipMemcpyAsync(h, st.iters, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
     g_syn_err = "synthetic solve: reading the loop state failed";
@@ -812,6 +817,10 @@ int synthetic_unfinished(SyntheticState &st, hipStream_t s, bool *unfinished, in
   }
   *unfinished = h[2] != 0 && h[1] == 0;
   *iters = h[0];
+  *bad_t = h[1] != 0;
+  double r;
+  memcpy(&r, h + 4, sizeof(r));
+  *res = r;
   return 0;
 }
 

@@ -93,6 +93,60 @@ module gsis_capi
       type(c_ptr), value :: h
       real(c_double), intent(out) :: frd(*)
     end function
+    ! the uniform Maxwellian fill of GKUA_sudufenbu on the device (no FRD array)
+    integer(c_int) function gsis_fill_frd_uniform(h, g, hv) bind(C, name="gsis_fill_frd_uniform")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: g(*), hv(*)
+    end function
+    ! the learned HoT closure in the synthetic solve (net from gsis_mlp_create, or c_null_ptr)
+    integer(c_int) function gsis_set_closure(h, net, gate, every) bind(C, name="gsis_set_closure")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h, net
+      real(c_double), value :: gate
+      integer(c_int32_t), value :: every
+    end function
+    integer(c_int) function gsis_mlp_create(net) bind(C, name="gsis_mlp_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: net
+    end function
+    integer(c_int) function gsis_mlp_set_params(net, params) bind(C, name="gsis_mlp_set_params")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: net
+      real(c_float), intent(in) :: params(*)
+    end function
+    integer(c_int) function gsis_mlp_set_scaling(net, scaling) bind(C, name="gsis_mlp_set_scaling")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: net
+      real(c_float), intent(in) :: scaling(*)
+    end function
+    integer(c_int) function gsis_mlp_destroy(net) bind(C, name="gsis_mlp_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: net
+    end function
+    ! batches of handles (hs: an array of handles)
+    integer(c_int) function gsis_batch_create(hs, n, b) bind(C, name="gsis_batch_create")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), intent(in) :: hs(*)
+      integer(c_int32_t), value :: n
+      type(c_ptr), intent(out) :: b
+    end function
+    integer(c_int) function gsis_batch_step(b, nsteps) bind(C, name="gsis_batch_step")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: nsteps
+    end function
+    integer(c_int) function gsis_batch_tune(b, steps_per_trial, nw, order, ms_per_step) bind(C, name="gsis_batch_tune")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: steps_per_trial
+      integer(c_int32_t), intent(out) :: nw, order
+      real(c_double), intent(out) :: ms_per_step
+    end function
+    integer(c_int) function gsis_batch_destroy(b) bind(C, name="gsis_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
     integer(c_int) function gsis_counters(h, kstep, tt, ns_iter) bind(C, name="gsis_counters")
       import :: c_int, c_ptr, c_int32_t, c_double
       type(c_ptr), value :: h

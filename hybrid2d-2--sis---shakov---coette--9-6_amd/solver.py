@@ -41,11 +41,19 @@ class GsisSolver:
     def refresh_moments(self):
         capi.check(self.lib, self.lib.gsis_refresh_moments(self.h))
 
-    def initialize(self):
-        """Initial state as init_flow_zero + GKUA_sudufenbu + the NS ghost BCs."""
-        frd, s, cf, u = initial_state(self.case)
-        self.upload(frd=frd, s=s, cf=cf, u=u, u0=u if self.mode == capi.MODE_M1 else None)
-        del frd
+    def initialize(self, device_fill: bool = True):
+        """Initial state as init_flow_zero + GKUA_sudufenbu + the NS ghost BCs.
+        The initial distribution is the same Maxwellian in every cell: with
+        ``device_fill`` the device writes it (gsis_fill_frd_uniform) instead of
+        uploading the full FRD array."""
+        gh, s, cf, u = initial_state(self.case, with_frd=not device_fill)
+        if device_fill:
+            g, hv = (np.ascontiguousarray(a, dtype=np.float64) for a in gh)
+            capi.check(self.lib, self.lib.gsis_fill_frd_uniform(self.h, _dp(g), _dp(hv)))
+            self.upload(s=s, cf=cf, u=u, u0=u if self.mode == capi.MODE_M1 else None)
+        else:
+            self.upload(frd=gh, s=s, cf=cf, u=u, u0=u if self.mode == capi.MODE_M1 else None)
+        del gh
         self.refresh_moments()
         m = self.macro()
         u = ns_boundary_init(self.case, m["u"], m["swall"])

@@ -3,15 +3,16 @@ the GPU, and evaluate it on held-out cavities.
 
     python hybrid2d-2--sis---shakov---coette--9-6_amd/train_closure.py --out closure_hot.npz --report report.json
 
-Training set: the planar Couette flow (4 x 128 cells, 32^2 velocities,
-walls +-0.1) at Kn 0.01, 0.03, 0.1, 0.3 and 1, and the lid-driven cavity
-(128^2 cells, 64^2 velocities, lid 0.15) at Kn 0.01, 0.1 and 1, each run in
-GSIS (mode M1) with first-order upwind fluxes (the scheme whose cavity runs
-converge; with the NND limiter their residual stalls); after every step the
-network inputs and the Convert_NS HoT of all closure faces are kept.  Held
-out: cavities with Kn drawn log-uniformly from [0.005, 2].  Training runs
-gsis_mlp_train (BF16-MFMA forward and backward, Adam) on the standardised
-samples.
+Training set: states near convergence (the last `keep` of `steps` GSIS
+steps, mode M1, first-order upwind fluxes, the scheme whose cavity runs
+converge; with the NND limiter their residual stalls) of the planar Couette
+flow (4 x 128 cells, 32^2 velocities, walls +-0.1) at Kn 0.01, 0.03, 0.1,
+0.3 and 1, and of the lid-driven cavity (lid 0.15) on 128^2 cells with 64^2
+velocities at Kn 0.01, 0.1 and 1 and on 64^2 cells with 32^2 velocities at
+Kn 0.02, 0.05, 0.2 and 0.5.  Each sample is one closure face: its network
+inputs and the Convert_NS HoT.  Held out: cavities with Kn drawn
+log-uniformly from [0.005, 2].  Training runs gsis_mlp_train (BF16-MFMA
+forward and backward, Adam) on the standardised samples.
 """
 from __future__ import annotations
 
@@ -68,11 +69,16 @@ def upwind(pkg, case):
     return case
 
 
-def training_cases(pkg, steps: int = 30):
+def training_cases(pkg, steps: int, keep: int):
+    """(case, steps, first kept step) of every training run."""
     cou = [upwind(pkg, pkg.cases.couette_case(ny_cells=128, nx_cells=4, nv=32, kn=kn, uwall=0.1, dt=1.0))
            for kn in (0.01, 0.03, 0.1, 0.3, 1.0)]
-    cav = [upwind(pkg, pkg.cases.cavity_case(128, 64, kn=kn, lid=0.15, dt=1.0)) for kn in (0.01, 0.1, 1.0)]
-    return [*[(c, 60) for c in cou], *[(c, steps) for c in cav]]
+    big = [upwind(pkg, pkg.cases.cavity_case(128, 64, kn=kn, lid=0.15, dt=1.0)) for kn in (0.01, 0.1, 1.0)]
+    small = [upwind(pkg, pkg.cases.cavity_case(64, 32, kn=kn, lid=0.15, dt=1.0)) for kn in (0.02, 0.05, 0.2, 0.5)]
+    runs = [(c, 100, 61) for c in cou]
+    runs += [(c, steps, steps - keep + 1) for c in big]
+    runs += [(c, 2 * steps, 2 * steps - 2 * keep + 1) for c in small]
+    return runs
 
 
 def heldout_kn(count: int, seed: int):
@@ -100,21 +106,24 @@ def main(argv=None):
     ap.add_argument("--gate", type=float, default=1.0)
     ap.add_argument("--heldout", type=int, default=4)
     ap.add_argument("--heldout-n", type=int, default=64)
-    ap.add_argument("--heldout-steps", type=int, default=30)
-    ap.add_argument("--steps", type=int, default=30, help="GSIS steps of each training cavity")
+    ap.add_argument("--heldout-steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=60, help="GSIS steps of each 128^2 training cavity (twice that at 64^2)")
+    ap.add_argument("--keep", type=int, default=20, help="last steps of each 128^2 run kept (twice that at 64^2)")
+    ap.add_argument("--ns-maxit", type=int, default=1000)
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the training steps (quick runs)")
     args = ap.parse_args(argv)
     pkg = _pkg()
     from gsis_amd import closure
     t0 = time.time()
     xs, ys, runs = [], [], {}
-    for case, steps in training_cases(pkg, args.steps):
+    for case, steps, first in training_cases(pkg, args.steps, args.keep):
         n = max(2, int(round(steps * args.scale)))
-        print(f"harvest {case.name}: {n} GSIS steps", flush=True)
-        x, y, res = harvest(case, n)
+        f = max(1, min(n, int(round(first * args.scale))))
+        print(f"harvest {case.name}: {n} GSIS steps, kept from step {f}", flush=True)
+        x, y, res = harvest(case, n, first=f, ns_maxit=args.ns_maxit)
         xs.append(x)
         ys.append(y)
-        runs[case.name] = {"steps": n, "samples": int(x.shape[0]), "history": res}
+        runs[case.name] = {"steps": n, "first_kept": f, "samples": int(x.shape[0]), "history": res}
     x = np.concatenate(xs)
     y = np.concatenate(ys)
     ok = np.all(np.isfinite(x), axis=1) & np.all(np.isfinite(y), axis=1)
@@ -140,7 +149,7 @@ def main(argv=None):
     held = {}
     for kn in heldout_kn(args.heldout, args.seed):
         case = upwind(pkg, pkg.cases.cavity_case(args.heldout_n, 32, kn=kn, lid=0.15, dt=1.0))
-        hx, hy, _ = harvest(case, args.heldout_steps, first=args.heldout_steps)
+        hx, hy, _ = harvest(case, args.heldout_steps, first=args.heldout_steps, ns_maxit=args.ns_maxit)
         e, a = evaluate(net, hx, hy, args.gate)
         held[f"{kn:.4g}"] = {"rel_l2": e, "accepted": a, "faces": int(hx.shape[0])}
         print(f"held-out Kn {kn:.4g}: rel L2 {np.round(e, 4).tolist()}, accepted {a:.3f}", flush=True)

@@ -271,11 +271,48 @@ int gsis_closure_samples(gsis_handle *h, float *x, float *y, int64_t cap, int64_
  * the fixed point is the one without the network.  A face is accepted when
  * the network reproduces its DVM HoT at the step start within `gate`
  * standard deviations of the training targets in every output; the others
- * keep the DVM HoT (the fallback).  net NULL switches the closure off.  The
- * network must outlive its use and is not used concurrently by two handles. */
+ * keep the DVM HoT (the fallback).  Residual check: when a launch after an
+ * update ends with T < 0.1 or with max |Res_4| more than 1.5 times that of
+ * the launch before, the step falls back to the DVM HoT on every face and
+ * continues the inner loop from its starting state.  net NULL switches the
+ * closure off.  The network must outlive its use and is not used
+ * concurrently by two handles. */
 int gsis_set_closure(gsis_handle *h, gsis_mlp *net, double gate, int32_t every);
-/* Closure faces, accepted faces and HoT refreshes of the last step. */
-int gsis_closure_stats(gsis_handle *h, int64_t *faces, int64_t *accepted, int32_t *refreshes);
+/* Closure faces, accepted faces and HoT refreshes of the last step, and
+ * whether its residual check fell back to the DVM HoT (1) or not (0). */
+int gsis_closure_stats(gsis_handle *h, int64_t *faces, int64_t *accepted, int32_t *refreshes, int32_t *rejected);
+
+/* ---- Initial distribution and batches of instances ----------------------
+ * No counterparts in the reference. */
+
+/* The Maxwellian fill of GKUA_sudufenbu (src/GKUA_sudufenbu.f90:22-46) for a
+ * uniform initial state, on the device: every cell's G and H set to g[v] and
+ * h[v] (nvi*nvj values each, velocity iv fastest), the wall slots zero; the
+ * same device state as gsis_upload_frd of the FRD array that fill makes,
+ * without the host array. */
+int gsis_fill_frd_uniform(gsis_handle *h, const double *g, const double *hv);
+
+/* A batch steps n handles together: the kinetic step of all instances is one
+ * launch of the column-wavefront kernel over every instance's strips, and
+ * the step of all instances up to the synthetic solve (S0, kinetic BCs, that
+ * launch, moments, DETFPwall, Convert_NS, HoT) is captured once as a hipGraph
+ * and replayed per step.  Instances in mode M1 then run their synthetic
+ * solve (with the closure when one is set), one after another; it is not
+ * captured (a cooperative launch whose iteration count the host controls).
+ * Instances may differ in mesh, velocity grid, gas, walls and mode; they
+ * must share precision, flux scheme and device, and fit the
+ * column-wavefront kernel.  A batch step keeps no kinetic RHS (gsis_residual
+ * reports the last one a gsis_step kept).  The handles must not be stepped
+ * on their own while a gsis_batch_step runs; they may be between calls. */
+typedef struct gsis_batch gsis_batch;
+int gsis_batch_create(gsis_handle *const *hs, int32_t n, gsis_batch **out);
+int gsis_batch_step(gsis_batch *b, int32_t nsteps);
+/* Autotuner: runs steps_per_trial real batch steps with each strip width
+ * (8 or 16 columns per workgroup) and job order (as given, or the largest
+ * kinetic work first), keeps the fastest (every choice computes the same
+ * step) and reports it with its time per step. */
+int gsis_batch_tune(gsis_batch *b, int32_t steps_per_trial, int32_t *nw, int32_t *order, double *ms_per_step);
+int gsis_batch_destroy(gsis_batch *b);
 
 #ifdef __cplusplus
 }
