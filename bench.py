@@ -147,6 +147,147 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": n * 8, "copies": 5}), flush=True)
 
 
+def bench_batch(pkg, solver, args):
+    """The batched configuration: `instances` Couette and cavity instances
+    drawn from `seed` (mesh side 64-256, Kn 1e-3-10, wall or lid speed
+    0.05-0.3, velocity grid 24^2-64^2), FP32 sweeps with FP64 moments, the
+    kinetic step of all of them per graph replay.  The autotuner's trials
+    (4 candidates x 1 step) come before the warm-up steps."""
+    batch = _load("gsis_amd.batch", PKG_DIR / "batch.py")
+    t0 = time.perf_counter()
+    cases = batch.instances(args.instances, seed=args.seed)
+    solvers = []
+    for c in cases:
+        s = solver.GsisSolver(c, precision=pkg.capi.PREC_FP32, kstep_show=1 << 30)
+        s.initialize()
+        solvers.append(s)
+    b = batch.Batch(solvers)
+    units = sum(c.ncell * c.nv for c in cases)
+    ring = sum((c.nx + 1) * (c.ny + 1) * c.nv for c in cases)
+    print(f"# {len(cases)} instances, {units:.4g} cell-velocities, set up in {time.perf_counter() - t0:.1f} s",
+          flush=True)
+    nw, order, tune_ms = b.tune(1)
+    print(f"# autotuner: {nw} columns per workgroup, order {order}, {tune_ms:.2f} ms per step", flush=True)
+    b.step(args.warmup)
+    solvers[0].counters()
+    t1 = time.perf_counter()
+    b.step(args.steps)
+    el = time.perf_counter() - t1
+    ms = el / args.steps * 1e3
+    alg = (16 * units + 8 * ring) / (ms * 1e-3) / 1e9  # FP32: read+write G,H once; moments read G,H
+    ncou = sum("couette" in c.name for c in cases)
+    out = {
+        "metric": "cell·velocity updates/s (batched instances)",
+        "value": units * args.steps / el,
+        "unit": "cell·velocity updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-sweep/f64-moments", "data": "synthetic",
+        "config": {"workload": f"batch of {len(cases)} instances from seed {args.seed}: {ncou} planar Couette "
+                               f"(4 x side cells) and {len(cases) - ncou} lid-driven cavities (side^2 cells), side "
+                               "64-256, Kn log-uniform 1e-3-10, wall/lid speed 0.05-0.3, velocity grid 24^2-64^2, "
+                               "NND fluxes, kinetic step (mode M0), one hipGraph replay per step",
+                   "instances": len(cases), "cell_velocities": units},
+        "autotuner": {"columns_per_workgroup": nw, "order": "largest first" if order else "as drawn",
+                      "ms_per_step": tune_ms},
+        "roofline": {"bound": "hbm", "kernel": "batched step", "achieved": alg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / HBM_PEAK_GBS, "traffic": None,
+                     "note": "algorithmic bytes of the batched kinetic launch and the moments over the step time"},
+    }
+    b.close()
+    for s in solvers:
+        s.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_production(pkg, solver, args):
+    """Time to 1e-4 relative L2 (rho, u, T, sigma_xy, q) of plain FP64 GSIS
+    (synthetic solve to ns_tol every step, the reference's control) and of the
+    fast form (FP32 sweeps, at most --fast-maxit inner iterations per step,
+    the batched graph-replayed kinetic step, optionally the learned closure),
+    both against the last step of the plain FP64 run.  First-order upwind
+    fluxes: with NND the cavity's residual stalls.  Only the stepping is
+    timed; the field checks between steps are not."""
+    batch = _load("gsis_amd.batch", PKG_DIR / "batch.py")
+    closure = _load("gsis_amd.closure", PKG_DIR / "closure.py")
+    case = make_case(pkg, args.n, args.nv, args.kn, args.lid, args.dt)
+    case.const["scheme"] = pkg.capi.SCHEME_UP1
+    nx, ny = case.nx, case.ny
+
+    def run(prec, maxit, steps, use_batch, use_closure, ref=None, tol=1e-4, cfl=20.0, dtmax=10.0):
+        case.const["cfl"] = cfl
+        case.const["dtmax"] = dtmax
+        s = solver.GsisSolver(case, precision=prec, mode=pkg.capi.MODE_M1, ns_maxit=maxit, kstep_show=1 << 30)
+        s.initialize()
+        b = batch.Batch([s]) if use_batch else None
+        net = None
+        if use_closure:
+            net = closure.trained_network()
+            closure.set_closure(s, net, gate=1.0, every=max(1, min(maxit, 50)) if maxit else 50)
+        snaps, t, its, hit = [], 0.0, [], None
+        for k in range(1, steps + 1):
+            t0 = time.perf_counter()
+            if b is not None:
+                b.step(1)
+            else:
+                s.step(1)
+            its.append(s.counters()[2])
+            t += time.perf_counter() - t0
+            S = s.macro()["s"]
+            if ref is None:
+                snaps.append((k, t, S.copy()))
+            else:
+                e = max(pkg.fields.field_errors(S, ref, nx, ny).values())
+                if hit is None and e < tol:
+                    hit = (k, t, e)
+                    break
+            if k % 10 == 0:
+                print(f"# {'fp32' if prec else 'fp64'} maxit {maxit}: step {k}, {t:.1f} s, inner its {its[-1]}",
+                      flush=True)
+        if b is not None:
+            b.close()
+        s.close()
+        return snaps, t, its, hit
+
+    units = case.ncell * case.nv
+    snaps, t_ref, its_ref, _ = run(pkg.capi.PREC_FP64, 0, args.ref_steps, False, False)
+    ref = snaps[-1][2]
+    hit_plain = None
+    for k, t, S in snaps:
+        e = max(pkg.fields.field_errors(S, ref, nx, ny).values())
+        if e < 1e-4:
+            hit_plain = (k, t, e)
+            break
+    _, t_fast, its_fast, hit_fast = run(pkg.capi.PREC_FP32, args.fast_maxit, args.ref_steps, True, bool(args.closure),
+                                        ref=ref, cfl=args.fast_cfl, dtmax=args.fast_dtmax)
+
+    def leg(hit, its, esz):
+        if hit is None:
+            return {"reached": False}
+        k, t, e = hit
+        # algorithmic bytes of the kinetic kernels per step (read+write G,H;
+        # moments read G,H) and of the inner loop (64 B per swept cell and iteration)
+        swept = max(nx - 5, 0) * max(ny - 5, 0)
+        byt = k * (6 * esz * units) + 64 * swept * sum(its[:k])
+        return {"reached": True, "steps": k, "seconds": t, "rel_l2": e, "inner_iterations": int(sum(its[:k])),
+                "achieved_GBs": byt / t / 1e9, "frac_of_hbm": byt / t / 1e9 / HBM_PEAK_GBS}
+
+    out = {
+        "metric": "time to 1e-4 rel L2 vs plain FP64 GSIS",
+        "config": {"workload": f"lid-driven cavity {args.n}x{args.n} cells, {args.nv}x{args.nv} sinh velocity grid, "
+                               f"Kn={args.kn}, lid {args.lid}, dt={args.dt}, UP1 fluxes, from rest; reference = step "
+                               f"{args.ref_steps} of the plain FP64 run"},
+        "plain_fp64": leg(hit_plain, its_ref, 8),
+        "fast": dict(leg(hit_fast, its_fast, 4), precision="fp32 sweeps / fp64 moments and synthetic solve",
+                     inner_iterations_cap=args.fast_maxit, cfl=args.fast_cfl, dtmax=args.fast_dtmax,
+                     batched_graph=True, closure=bool(args.closure)),
+        "plain_run": {"steps": args.ref_steps, "seconds": t_ref},
+    }
+    if hit_plain and hit_fast:
+        out["speedup"] = hit_plain[1] / hit_fast[1]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +310,19 @@ def main():
     ap.add_argument("--tiles", type=int, default=0, help="velocity tiles per workgroup of the tile walker (0: auto)")
     ap.add_argument("--converge", type=int, default=0,
                     help="1: also measure time-to-converge on a cavity (first-order upwind), 0: skip")
+    ap.add_argument("--workload", choices=("cavity", "batch", "production"), default="cavity",
+                    help="cavity: one lid-driven cavity (the headline); batch: the heterogeneous batch of "
+                         "--instances Couette and cavity instances (FP32 sweeps, kinetic step, graph replays); "
+                         "production: time to 1e-4 relative L2 of plain FP64 GSIS and of the fast form")
+    ap.add_argument("--ref-steps", type=int, default=240, help="production: steps of the plain FP64 GSIS run")
+    ap.add_argument("--fast-maxit", type=int, default=0,
+                    help="production: cap on inner iterations per step of the fast form (0: none)")
+    ap.add_argument("--fast-cfl", type=float, default=20.0, help="production: CFL of the fast form's synthetic solve")
+    ap.add_argument("--fast-dtmax", type=float, default=10.0,
+                    help="production: largest local time step of the fast form's synthetic solve")
+    ap.add_argument("--closure", type=int, default=0, help="production: 1 uses the trained closure in the fast form")
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy a device buffer of BYTES bytes 5 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -182,6 +336,12 @@ def main():
         return
     pkg = _load("gsis_amd", PKG_DIR / "__init__.py", PKG_DIR)
     solver = _load("gsis_amd.solver", PKG_DIR / "solver.py")
+    if args.workload == "batch":
+        bench_batch(pkg, solver, args)
+        return
+    if args.workload == "production":
+        bench_production(pkg, solver, args)
+        return
     case = make_case(pkg, args.n, args.nv, args.kn, args.lid, args.dt)
     case.const["scheme"] = {"nnd": pkg.capi.SCHEME_NND, "weno3": pkg.capi.SCHEME_WENO3,
                             "up1": pkg.capi.SCHEME_UP1}[args.scheme]

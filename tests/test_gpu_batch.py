@@ -73,3 +73,31 @@ def test_batch_equals_single_instances(prec):
     b.close()
     for s in single + grouped:
         s.close()
+
+
+def test_batch_runs_m1_instances_like_single_steps():
+    """M1 instances in a batch: the captured part of the step, then each
+    instance's synthetic solve; the same fields and inner iteration counts as
+    stepping each instance on its own."""
+    g = load_pkg()
+    from gsis_amd import batch
+    cases = batch.instances(3, seed=7, side=(12, 24), nv=(8, 16), precision_scheme=g.capi.SCHEME_UP1)
+    single, grouped = [], []
+    for c in cases:
+        for lst in (single, grouped):
+            s = solver_mod().GsisSolver(c, mode=g.capi.MODE_M1, kstep_show=1000000)
+            s.initialize()
+            lst.append(s)
+    b = batch.Batch(grouped)
+    for k in range(3):
+        b.step(1)
+        for s, t in zip(single, grouped):
+            s.step(1)
+            assert s.counters() == t.counters(), (s.case.name, k)
+    for s, t in zip(single, grouped):
+        ms, mt = s.macro(), t.macro()
+        for key in KEYS:
+            assert np.array_equal(ms[key], mt[key]), (s.case.name, key)
+    b.close()
+    for s in single + grouped:
+        s.close()

@@ -99,3 +99,32 @@ def test_closure_samples_are_convert_ns_hot_and_face_gradients():
             r0, t0 = (c(rho, i, j) + c(rho, i, j - 1)) / 2, (c(T, i, j) + c(T, i, j - 1)) / 2
         want = np.array([ux, vy, uy + vx, tx, ty, r0, t0, math.log10(kn)])
         assert np.allclose(x[f], want, rtol=1e-5, atol=1e-6 * (1 + np.abs(want).max())), (d, i, j, x[f], want)
+
+
+def test_wrong_network_is_rejected_and_step_equals_plain():
+    """A network whose HoT updates wreck the inner loop trips the residual
+    check on every step: the step falls back to the DVM HoT and the loop's
+    starting state, and the fields equal those of the plain step."""
+    g = load_pkg()
+    from gsis_amd import closure
+    case = g.cases.cavity_case(16, 24, kn=0.1, lid=0.15, dt=1.0)
+    plain = _m1(case)
+    guarded = _m1(case)
+    rng = np.random.default_rng(3)
+    p = (closure.init_params(5) * 4.0).astype(np.float32)
+    scaling = np.concatenate([np.zeros(8), np.full(8, 1e-3), np.zeros(5), np.full(5, 50.0)]).astype(np.float32)
+    net = closure.Mlp(p, scaling)
+    closure.set_closure(guarded, net, gate=1e30, every=5)
+    rejected = 0
+    for k in range(3):
+        plain.step(1)
+        guarded.step(1)
+        st = closure.stats(guarded)
+        print("step", k + 1, st, "plain its", plain.counters()[2], "guarded its", guarded.counters()[2], flush=True)
+        rejected += st["rejected"]
+    assert rejected >= 1
+    a, b = plain.macro(), guarded.macro()
+    for key in ("s", "cf", "u", "taoi", "taoj", "qi", "qj"):
+        assert np.array_equal(a[key], b[key]), key
+    plain.close()
+    guarded.close()
