@@ -219,6 +219,8 @@ def bench_production(pkg, solver, args):
         case.const["dtmax"] = dtmax
         s = solver.GsisSolver(case, precision=prec, mode=pkg.capi.MODE_M1, ns_maxit=maxit, kstep_show=1 << 30)
         s.initialize()
+        if args.mass_correct:
+            s.set_mass_correct(True)
         b = batch.Batch([s]) if use_batch else None
         net = None
         if use_closure:
@@ -282,6 +284,7 @@ def bench_production(pkg, solver, args):
                      inner_iterations_cap=args.fast_maxit, cfl=args.fast_cfl, dtmax=args.fast_dtmax,
                      batched_graph=True, closure=bool(args.closure)),
         "plain_run": {"steps": args.ref_steps, "seconds": t_ref},
+        "mass_correct": bool(args.mass_correct),
     }
     if hit_plain and hit_fast:
         out["speedup"] = hit_plain[1] / hit_fast[1]
@@ -321,6 +324,9 @@ def main():
     ap.add_argument("--fast-dtmax", type=float, default=10.0,
                     help="production: largest local time step of the fast form's synthetic solve")
     ap.add_argument("--closure", type=int, default=0, help="production: 1 uses the trained closure in the fast form")
+    ap.add_argument("--mass-correct", type=int, default=1,
+                    help="production: 1 runs both forms with the reference's MassCorrect after the moments "
+                         "(without it GSIS loses mass in the closed cavity every step and does not converge)")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",

@@ -59,6 +59,7 @@ EXPORTED = (
     "gsis_timing",
     "gsis_destroy",
     "gsis_last_error",
+    "gsis_set_mass_correct",
     "gsis_closure_samples",
     "gsis_set_closure",
     "gsis_closure_stats",
@@ -187,6 +188,7 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.gsis_last_error.restype = ctypes.c_char_p
     c_fp = ctypes.POINTER(ctypes.c_float)
     c_i64p = ctypes.POINTER(ctypes.c_int64)
+    lib.gsis_set_mass_correct.argtypes = [h, ctypes.c_int32]
     lib.gsis_closure_samples.argtypes = [h, c_fp, c_fp, ctypes.c_int64, c_i64p]
     lib.gsis_set_closure.argtypes = [h, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]
     lib.gsis_closure_stats.argtypes = [h, c_i64p, c_i64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]

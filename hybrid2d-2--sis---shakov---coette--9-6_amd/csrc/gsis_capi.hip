@@ -71,6 +71,10 @@ struct gsis_handle {
   float cl_gate = 0.0f;
   int cl_every = 0;
   gsis::ClosureBufs cl{};
+  // MassCorrect (gsis_set_mass_correct): on/off, Init_mass, out_mass on the device
+  bool mass_on = false;
+  double init_mass = 0.0;
+  double *mwork = nullptr;
   // optional kernel timing (gsis_timing): events around the kinetic kernels
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -268,6 +272,8 @@ int one_step(gsis_handle *h) {
   phy_boundary<T>(h);
   kinetic<T>(h);
   moments<T>(h, true);
+  // the place of the reference's commented-out call MassCorrect (src/Hybrid_Sub.f90:42)
+  if (h->mass_on) gsis::launch_mass_correct(h->d, h->mwork, h->init_mass, h->s);
   gsis::launch_convert(h->d, h->s);
   if (h->p.if_viscous) gsis::launch_hot(h->d, h->s);
   if (h->p.mode == GSIS_MODE_M1) {
@@ -467,6 +473,7 @@ int batch_capture(gsis_batch *b) {
                                 b->nseam, b->ncellb, b->epoch, b->epoch + 1, b->s);
     for (gsis_handle *h : b->hs) {
       moments<T>(h);
+      if (h->mass_on) gsis::launch_mass_correct(h->d, h->mwork, h->init_mass, b->s);
       gsis::launch_convert(h->d, b->s);
       if (h->p.if_viscous) gsis::launch_hot(h->d, b->s);
     }
@@ -736,6 +743,24 @@ int gsis_closure_samples(gsis_handle *h, float *x, float *y, int64_t cap, int64_
   if (e == hipSuccess) e = hipStreamSynchronize(h->s);
   (void)hipFree(dx);
   if (e != hipSuccess) return fail(GSIS_EHIP, std::string("closure samples: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int gsis_set_mass_correct(gsis_handle *h, int32_t on) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  if (!on) {
+    h->mass_on = false;
+    return 0;
+  }
+  if (!h->mwork) {
+    int rc = dalloc(h, &h->mwork, 64);
+    if (rc) return rc;
+  }
+  gsis::launch_out_mass(h->d, h->mwork, h->s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&h->init_mass, h->mwork, sizeof(double), hipMemcpyDeviceToHost, h->s));
+  HIPCHK(hipStreamSynchronize(h->s));
+  h->mass_on = true;
   return 0;
 }
 

@@ -86,6 +86,10 @@ void launch_frd_out_chunk(const Dev &d, double *stage, int v0, int nvc, const T 
 template <typename T>
 void launch_frd_in_chunk(const Dev &d, const double *stage, int v0, int nvc, T *FG, T *FH, T *slot, hipStream_t s);
 void launch_save_s0(const Dev &d, hipStream_t s);
+// out_mass (sum of S_GKUA(1)*vol over interior cells) into out[0]
+void launch_out_mass(const Dev &d, double *out, hipStream_t s);
+// MassCorrect: out_mass into work[0], then S_GKUA(1) scaled by init/out_mass
+void launch_mass_correct(const Dev &d, double *work, double init, hipStream_t s);
 void launch_det_wall(const Dev &d, const int *list, int n, double *swall, hipStream_t s);
 void launch_convert(const Dev &d, hipStream_t s);
 void launch_hot(const Dev &d, hipStream_t s);

@@ -2295,6 +2295,44 @@ void launch_hot(const Dev &d, hipStream_t s) {
   const int n = d.nx * (d.ny - 1) + (d.nx - 1) * d.ny;
   hipLaunchKernelGGL(k_hot, dim3((n + 127) / 128), dim3(128), 0, s, d);
 }
+// out_mass of DETFP (src/GKUA_DETFP.f90:134-138): the sum of S_GKUA(1)*vol
+// over the interior cells.  One workgroup, a fixed summation order: the same
+// value on every run.
+__global__ void __launch_bounds__(1024) k_out_mass(Dev d, double *out) {
+  __shared__ double red[1024];
+  const int ncol = d.nx - 1, n = ncol * (d.ny - 1);
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < n; t += 1024) {
+    const int i = t % ncol + 1, j = t / ncol + 1;
+    acc += d.S[cidx(d, i, j)] * d.vol[gidx(d, i, j)];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = 512; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// MassCorrect (src/GKUA_DETFP.f90:153-172): S_GKUA(1) of every interior cell
+// times Init_mass/out_mass, evaluated as (S*Init_mass)/out_mass
+__global__ void k_mass_scale(Dev d, const double *out, double init) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncol = d.nx - 1;
+  if (t >= ncol * (d.ny - 1)) return;
+  const int c = cidx(d, t % ncol + 1, t / ncol + 1);
+  d.S[c] = d.S[c] * init / out[0];
+}
+
+void launch_out_mass(const Dev &d, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_out_mass, dim3(1), dim3(1024), 0, s, d, out);
+}
+void launch_mass_correct(const Dev &d, double *work, double init, hipStream_t s) {
+  launch_out_mass(d, work, s);
+  const int n = (d.nx - 1) * (d.ny - 1);
+  hipLaunchKernelGGL(k_mass_scale, dim3((n + 255) / 256), dim3(256), 0, s, d, (const double *)work, init);
+}
 void launch_residual(const Dev &d, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_residual, dim3(1), dim3(1024), 0, s, d, out);
 }

@@ -99,6 +99,12 @@ module gsis_capi
       type(c_ptr), value :: h
       real(c_double), intent(in) :: g(*), hv(*)
     end function
+    ! MassCorrect (src/GKUA_DETFP.f90:153-172) after the moments of every later step
+    integer(c_int) function gsis_set_mass_correct(h, on) bind(C, name="gsis_set_mass_correct")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: on
+    end function
     ! the learned HoT closure in the synthetic solve (net from gsis_mlp_create, or c_null_ptr)
     integer(c_int) function gsis_set_closure(h, net, gate, every) bind(C, name="gsis_set_closure")
       import :: c_int, c_ptr, c_double, c_int32_t

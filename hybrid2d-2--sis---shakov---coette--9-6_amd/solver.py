@@ -100,6 +100,11 @@ class GsisSolver:
         self.set_counters(kstep, tt)
         return slot
 
+    def set_mass_correct(self, on: bool = True) -> None:
+        """MassCorrect (src/GKUA_DETFP.f90:153-172) after the moments of every
+        later step, Init_mass taken from the current state."""
+        capi.check(self.lib, self.lib.gsis_set_mass_correct(self.h, int(bool(on))))
+
     def set_counters(self, kstep: int, tt: float) -> None:
         capi.check(self.lib, self.lib.gsis_set_counters(self.h, int(kstep), float(tt)))
 

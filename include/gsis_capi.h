@@ -215,6 +215,16 @@ int gsis_destroy(gsis_handle *h);
 
 const char *gsis_last_error(void);
 
+/* MassCorrect (src/GKUA_DETFP.f90:153-172), whose call the reference's step
+ * keeps commented out (src/Hybrid_Sub.f90:42): with on != 0, after the
+ * moments of every later step, S_GKUA(1) of every interior cell is scaled by
+ * Init_mass/out_mass, out_mass being the sum of S_GKUA(1)*vol over the
+ * interior cells (src/GKUA_DETFP.f90:134-138).  Init_mass is the out_mass of
+ * the current state, as the reference sets it after a restart
+ * (src/Backup_write_read.f90:102-104).  Off by default (the step as shipped);
+ * set it before creating a batch that holds the handle. */
+int gsis_set_mass_correct(gsis_handle *h, int32_t on);
+
 /* ---- Learned closure of the high-order terms (HoT) ------------------------
  * No counterpart in the reference.  Convert_NS computes the HoT that the
  * synthetic NS equations add to their stress and heat flux from the DVM

@@ -900,6 +900,30 @@ int orc_convert_ns(orc_t *o) {
 
 int orc_synthetic_ns(orc_t *o); /* defined in gsis_oracle_ns.c */
 
+/* out_mass of DETFP: the sum of S_GKUA(1)*vol over the interior cells, j
+   outer and i inner (src/GKUA_DETFP.f90:134-138) */
+double orc_out_mass(const orc_t *o) {
+  double m = 0.0;
+  for (int j = 1; j <= o->ny - 1; j++)
+    for (int i = 1; i <= o->nx - 1; i++) m = m + S_(1, i, j) * G2(o->vol, i, j);
+  return m;
+}
+
+/* MassCorrect on: Init_mass is the out_mass of the current state, as after a
+   restart (src/Backup_write_read.f90:102-104) */
+void orc_set_mass_correct(orc_t *o, int on) {
+  o->mass_correct = on;
+  if (on) o->init_mass = orc_out_mass(o);
+}
+
+/* MassCorrect (src/GKUA_DETFP.f90:153-172): S_GKUA(1) of every interior cell
+   scaled by Init_mass/out_mass */
+static void mass_correct(orc_t *o) {
+  const double out_mass = orc_out_mass(o);
+  for (int j = 1; j <= o->ny - 1; j++)
+    for (int i = 1; i <= o->nx - 1; i++) S_(1, i, j) = S_(1, i, j) * o->init_mass / out_mass;
+}
+
 /* Hybrid_Time_advance_LU_SGS, src/Hybrid_Sub.f90:4-160 */
 int orc_step(orc_t *o, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
@@ -909,6 +933,8 @@ int orc_step(orc_t *o, int nsteps) {
     orc_phy_boundary(o);
     kinetic_solve(o);
     orc_detfp(o);
+    /* the place of the reference's (commented-out) call MassCorrect, src/Hybrid_Sub.f90:42 */
+    if (o->mass_correct) mass_correct(o);
     int rc = orc_convert_ns(o);
     if (rc) return rc;
     if (o->mode == GSIS_MODE_M1) {

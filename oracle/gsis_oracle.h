@@ -32,6 +32,8 @@ void orc_detfp(orc_t *o);
 int orc_convert_ns(orc_t *o);
 int orc_step(orc_t *o, int nsteps);
 int orc_residual(orc_t *o, double res_rms[4], double res_max[4]);
+double orc_out_mass(const orc_t *o);
+void orc_set_mass_correct(orc_t *o, int on);
 double orc_ns_advance(orc_t *o, int n);
 void orc_ns_work(const orc_t *o, double *res, double *du, double *lc);
 #ifdef __cplusplus

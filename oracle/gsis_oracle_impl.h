@@ -54,6 +54,9 @@ struct orc {
   double *res, *du, *dtl, *lci, *lcj, *lvi, *lvj;
   int kstep, ns_iter;
   double tt;
+  /* MassCorrect (src/GKUA_DETFP.f90:153-172): on/off and Init_mass */
+  int mass_correct;
+  double init_mass;
 };
 
 /* shared internals (gsis_oracle.c) */

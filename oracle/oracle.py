@@ -63,6 +63,10 @@ def load():
         lib.orc_step.restype = ctypes.c_int
         lib.orc_residual.argtypes = [vp, dp, dp]
         lib.orc_residual.restype = ctypes.c_int
+        lib.orc_out_mass.argtypes = [vp]
+        lib.orc_out_mass.restype = ctypes.c_double
+        lib.orc_set_mass_correct.argtypes = [vp, ctypes.c_int]
+        lib.orc_set_mass_correct.restype = None
         lib.orc_ns_advance.argtypes = [vp, ctypes.c_int]
         lib.orc_ns_advance.restype = ctypes.c_double
         lib.orc_ns_work.argtypes = [vp, dp, dp, dp]
@@ -243,6 +247,12 @@ class OracleSolver:
         # each of the five is a Fortran (-1:nx+1,-1:ny+1) array
         lc = np.stack([np.asfortranarray(lc[k].ravel().reshape(sh, order="F")) for k in range(5)])
         return {"res": res, "du": du, "lci": lc[0], "lcj": lc[1], "lvi": lc[2], "lvj": lc[3], "dt": lc[4]}
+
+    def out_mass(self) -> float:
+        return float(self.lib.orc_out_mass(self.h))
+
+    def set_mass_correct(self, on: bool = True):
+        self.lib.orc_set_mass_correct(self.h, int(bool(on)))
 
     def step(self, n: int = 1):
         rc = self.lib.orc_step(self.h, int(n))

@@ -175,6 +175,22 @@ def test_m1_cavity_parity():
     assert rel_l2(gpu.residual(), cpu.residual()) <= TOL
 
 
+def test_m1_cavity_mass_correct_parity():
+    """M1 with the reference's MassCorrect after the moments of each step."""
+    g = load_pkg()
+    case = g.cases.cavity_case(16, 24, kn=0.1, lid=0.15, dt=1.0)
+    gpu, cpu = _pair(case, mode=1)
+    gpu.set_mass_correct(True)
+    cpu.set_mass_correct(True)
+    for k in range(4):
+        gpu.step(1)
+        cpu.step(1)
+        assert gpu.counters()[2] == cpu.counters()[2]
+    _compare(gpu, cpu, "cavity16 M1 mass-corrected step 4")
+    gpu.close()
+    cpu.close()
+
+
 def test_split_and_fused_paths_agree():
     """The fused single-pass kernel and the split RHS/sweep kernels compute the
     same step (both are checked against the oracle)."""

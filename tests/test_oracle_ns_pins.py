@@ -441,3 +441,47 @@ def test_ns_farfield_and_periodic_ghost_cells():
         for layer in (1, 2):
             np.testing.assert_array_equal(uc2[:, 1 - layer + 1, j + 1], uc[:, nx - layer + 1, j + 1])
             np.testing.assert_array_equal(uc2[:, nx - 1 + layer + 1, j + 1], uc[:, layer + 1, j + 1])
+
+
+def test_mass_correct_scales_density_to_init_mass():
+    """MassCorrect (src/GKUA_DETFP.f90:153-172): after the moments of a step
+    every interior S_GKUA(1) is the uncorrected value times
+    Init_mass/out_mass, so the interior mass equals Init_mass; Init_mass is
+    the out_mass of the state when the correction is switched on."""
+    from helpers import load_oracle, load_pkg
+    g = load_pkg()
+    case = g.cases.cavity_case(10, 8, kn=0.2, lid=0.15, dt=0.5)
+    orc = load_oracle()
+    a, b = orc.OracleSolver(case), orc.OracleSolver(case)
+    for o in (a, b):
+        o.initialize()
+    m = a.macro()
+    s = m["s"].copy()
+    s[0] *= 1.3  # a state whose mass the next moments will not reproduce
+    a.upload(s=s)
+    vol = case.mesh.vol[:-1, :-1]
+    want = float((s[0, 4:14, 4:14] * vol).sum())
+    a.set_mass_correct(True)
+    assert a.out_mass() == pytest.approx(want, rel=1e-14)
+    a.step(2)
+    b.step(2)
+    assert a.out_mass() == pytest.approx(want, rel=1e-13)
+    sb = b.macro()["s"][0, 4:14, 4:14]
+    out_b = b.out_mass()
+    assert out_b == pytest.approx(float((sb * vol).sum()), rel=1e-14)
+    assert abs(out_b / want - 1.0) > 0.1  # the uncorrected mass is far from Init_mass
+    # step 1 of two runs from the same state: the uncorrected run's density
+    # times the ratio is the corrected run's, cell by cell
+    a2, b2 = orc.OracleSolver(case), orc.OracleSolver(case)
+    for o in (a2, b2):
+        o.initialize()
+        o.upload(s=s)
+    a2.set_mass_correct(True)
+    a2.step(1)
+    b2.step(1)
+    s_a = a2.macro()["s"][0, 4:14, 4:14]
+    s_b = b2.macro()["s"][0, 4:14, 4:14]
+    ratio = want / b2.out_mass()
+    assert np.allclose(s_a, s_b * ratio, rtol=1e-14, atol=0)
+    for o in (a, b, a2, b2):
+        o.close()
