@@ -214,12 +214,13 @@ def bench_production(pkg, solver, args):
     case.const["scheme"] = pkg.capi.SCHEME_UP1
     nx, ny = case.nx, case.ny
 
-    def run(prec, maxit, steps, use_batch, use_closure, ref=None, tol=1e-4, cfl=20.0, dtmax=10.0):
+    def run(prec, maxit, steps, use_batch, use_closure, ref=None, tol=1e-4, cfl=20.0, dtmax=10.0, mode=None):
         case.const["cfl"] = cfl
         case.const["dtmax"] = dtmax
-        s = solver.GsisSolver(case, precision=prec, mode=pkg.capi.MODE_M1, ns_maxit=maxit, kstep_show=1 << 30)
+        mode = pkg.capi.MODE_M1 if mode is None else mode
+        s = solver.GsisSolver(case, precision=prec, mode=mode, ns_maxit=maxit, kstep_show=1 << 30)
         s.initialize()
-        if args.mass_correct:
+        if args.mass_correct and mode == pkg.capi.MODE_M1:
             s.set_mass_correct(True)
         b = batch.Batch([s]) if use_batch else None
         net = None
@@ -243,9 +244,9 @@ def bench_production(pkg, solver, args):
                 if hit is None and e < tol:
                     hit = (k, t, e)
                     break
-            if k % 10 == 0:
-                print(f"# {'fp32' if prec else 'fp64'} maxit {maxit}: step {k}, {t:.1f} s, inner its {its[-1]}",
-                      flush=True)
+            if k % (10 if mode == pkg.capi.MODE_M1 else 500) == 0:
+                print(f"# {'fp32' if prec else 'fp64'} mode {mode} maxit {maxit}: step {k}, {t:.1f} s, "
+                      f"inner its {its[-1]}", flush=True)
         if b is not None:
             b.close()
         s.close()
@@ -262,6 +263,12 @@ def bench_production(pkg, solver, args):
             break
     _, t_fast, its_fast, hit_fast = run(pkg.capi.PREC_FP32, args.fast_maxit, args.ref_steps, True, bool(args.closure),
                                         ref=ref, cfl=args.fast_cfl, dtmax=args.fast_dtmax)
+    # the kinetic step alone (no synthetic solve) to the same reference
+    dvm = {}
+    if args.dvm_steps > 0:
+        for tag, prec, use_b in (("dvm_fp64", pkg.capi.PREC_FP64, False), ("dvm_fp32_batched", pkg.capi.PREC_FP32, True)):
+            _, _, its_d, hit_d = run(prec, 0, args.dvm_steps, use_b, False, ref=ref, mode=pkg.capi.MODE_M0)
+            dvm[tag] = hit_d
 
     def leg(hit, its, esz):
         if hit is None:
@@ -286,8 +293,42 @@ def bench_production(pkg, solver, args):
         "plain_run": {"steps": args.ref_steps, "seconds": t_ref},
         "mass_correct": bool(args.mass_correct),
     }
+    for tag, hit in dvm.items():
+        out[tag] = leg(hit, [0] * (hit[0] if hit else 0), 8 if tag == "dvm_fp64" else 4)
+        if hit and hit_plain:
+            out[tag]["speedup_vs_plain_gsis"] = hit_plain[1] / hit[1]
     if hit_plain and hit_fast:
         out["speedup"] = hit_plain[1] / hit_fast[1]
+    print(json.dumps(out), flush=True)
+
+
+def bench_decks(pkg, solver, args, nsteps=2150, every=50):
+    """The reference's shipped cylinder and channel decks as shipped (NND,
+    dt_global = 0.01, mode M0), `nsteps` steps on the GPU: the residual every
+    `every` steps, and the step at which a run stops with T < 0.1."""
+    golden = ROOT / "tests" / "golden"
+    out = {"metric": "residual history of the shipped decks", "steps": nsteps, "decks": {}}
+    for name in ("channel", "cylinder"):
+        case = pkg.cases.shipped_case(golden / name, name)
+        rec = {}
+        for leg, make in [("gpu", lambda: solver.GsisSolver(case, kstep_show=every))]:
+            s = make()
+            s.initialize()
+            hist, stop = [], None
+            t0 = time.perf_counter()
+            for k in range(every, nsteps + 1, every):
+                try:
+                    s.step(every)
+                    r = [float(x) for x in s.residual()]
+                except Exception as e:  # T < 0.1: the reference's stop
+                    stop = {"before_step": k, "error": str(e)[:120]}
+                    break
+                hist.append((k, r))
+            rec[leg] = {"history": hist, "stopped": stop, "seconds": time.perf_counter() - t0}
+            print(f"# {name} {leg}: {len(hist) * every} steps, last {hist[-1] if hist else None}, stop {stop}",
+                  flush=True)
+            s.close()
+        out["decks"][name] = rec
     print(json.dumps(out), flush=True)
 
 
@@ -313,7 +354,7 @@ def main():
     ap.add_argument("--tiles", type=int, default=0, help="velocity tiles per workgroup of the tile walker (0: auto)")
     ap.add_argument("--converge", type=int, default=0,
                     help="1: also measure time-to-converge on a cavity (first-order upwind), 0: skip")
-    ap.add_argument("--workload", choices=("cavity", "batch", "production"), default="cavity",
+    ap.add_argument("--workload", choices=("cavity", "batch", "production", "decks"), default="cavity",
                     help="cavity: one lid-driven cavity (the headline); batch: the heterogeneous batch of "
                          "--instances Couette and cavity instances (FP32 sweeps, kinetic step, graph replays); "
                          "production: time to 1e-4 relative L2 of plain FP64 GSIS and of the fast form")
@@ -324,6 +365,8 @@ def main():
     ap.add_argument("--fast-dtmax", type=float, default=10.0,
                     help="production: largest local time step of the fast form's synthetic solve")
     ap.add_argument("--closure", type=int, default=0, help="production: 1 uses the trained closure in the fast form")
+    ap.add_argument("--dvm-steps", type=int, default=0,
+                    help="production: also run the kinetic step alone (FP64, and FP32 batched) up to this many steps")
     ap.add_argument("--mass-correct", type=int, default=1,
                     help="production: 1 runs both forms with the reference's MassCorrect after the moments "
                          "(without it GSIS loses mass in the closed cavity every step and does not converge)")
@@ -347,6 +390,9 @@ def main():
         return
     if args.workload == "production":
         bench_production(pkg, solver, args)
+        return
+    if args.workload == "decks":
+        bench_decks(pkg, solver, args)
         return
     case = make_case(pkg, args.n, args.nv, args.kn, args.lid, args.dt)
     case.const["scheme"] = {"nnd": pkg.capi.SCHEME_NND, "weno3": pkg.capi.SCHEME_WENO3,
