@@ -227,7 +227,10 @@ def bench_production(pkg, solver, args):
         net = None
         if use_closure:
             net = closure.trained_network()
-            closure.set_closure(s, net, gate=1.0, every=max(1, min(maxit, 50)) if maxit else 50)
+            if args.closure_warmup > 0:
+                closure.set_warmup(s, net, args.closure_warmup, every=50)
+            if args.closure:
+                closure.set_closure(s, net, gate=1.0, every=max(1, min(maxit, 50)) if maxit else 50)
         snaps, t, its, hit = [], 0.0, [], None
         for k in range(1, steps + 1):
             t0 = time.perf_counter()
@@ -262,8 +265,10 @@ def bench_production(pkg, solver, args):
         if e < 1e-4:
             hit_plain = (k, t, e)
             break
-    _, t_fast, its_fast, hit_fast = run(pkg.capi.PREC_FP32, args.fast_maxit, args.ref_steps, True, bool(args.closure),
-                                        ref=ref, cfl=args.fast_cfl, dtmax=args.fast_dtmax)
+    fprec = pkg.capi.PREC_FP32 if args.fast_precision == "fp32" else pkg.capi.PREC_FP64
+    _, t_fast, its_fast, hit_fast = run(fprec, args.fast_maxit, args.ref_steps, True,
+                                        bool(args.closure) or args.closure_warmup > 0, ref=ref, cfl=args.fast_cfl,
+                                        dtmax=args.fast_dtmax)
     # the kinetic step alone (no synthetic solve) to the same reference
     dvm = {}
     if args.dvm_steps > 0:
@@ -288,9 +293,11 @@ def bench_production(pkg, solver, args):
                                f"Kn={args.kn}, lid {args.lid}, dt={args.dt}, UP1 fluxes, from rest; reference = step "
                                f"{args.ref_steps} of the plain FP64 run"},
         "plain_fp64": leg(hit_plain, its_ref, 8),
-        "fast": dict(leg(hit_fast, its_fast, 4), precision="fp32 sweeps / fp64 moments and synthetic solve",
+        "fast": dict(leg(hit_fast, its_fast, 4 if args.fast_precision == "fp32" else 8),
+                     precision=("fp32 sweeps / fp64 moments and synthetic solve" if args.fast_precision == "fp32"
+                                else "fp64"),
                      inner_iterations_cap=args.fast_maxit, cfl=args.fast_cfl, dtmax=args.fast_dtmax,
-                     batched_graph=True, closure=bool(args.closure)),
+                     batched_graph=True, closure=bool(args.closure), closure_warmup_steps=args.closure_warmup),
         "plain_run": {"steps": args.ref_steps, "seconds": t_ref},
         "mass_correct": bool(args.mass_correct),
     }
@@ -366,6 +373,9 @@ def main():
     ap.add_argument("--fast-dtmax", type=float, default=10.0,
                     help="production: largest local time step of the fast form's synthetic solve")
     ap.add_argument("--closure", type=int, default=0, help="production: 1 uses the trained closure in the fast form")
+    ap.add_argument("--closure-warmup", type=int, default=0,
+                    help="production: the trained network's HoT replace the DVM's in the fast form's first N steps")
+    ap.add_argument("--fast-precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--dvm-steps", type=int, default=0,
                     help="production: also run the kinetic step alone (FP64, and FP32 batched) up to this many steps")
     ap.add_argument("--mass-correct", type=int, default=1,

@@ -62,6 +62,7 @@ EXPORTED = (
     "gsis_set_mass_correct",
     "gsis_closure_samples",
     "gsis_set_closure",
+    "gsis_set_closure_warmup",
     "gsis_closure_stats",
     "gsis_mlp_create",
     "gsis_mlp_nparams",
@@ -191,6 +192,7 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.gsis_set_mass_correct.argtypes = [h, ctypes.c_int32]
     lib.gsis_closure_samples.argtypes = [h, c_fp, c_fp, ctypes.c_int64, c_i64p]
     lib.gsis_set_closure.argtypes = [h, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]
+    lib.gsis_set_closure_warmup.argtypes = [h, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
     lib.gsis_closure_stats.argtypes = [h, c_i64p, c_i64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     lib.gsis_mlp_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
     lib.gsis_mlp_nparams.argtypes = []

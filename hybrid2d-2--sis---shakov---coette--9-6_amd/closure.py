@@ -146,6 +146,14 @@ def set_closure(solver, net: Mlp | None, gate: float = 1.0, every: int = 50) -> 
     solver._closure_net = net  # the network must outlive its use
 
 
+def set_warmup(solver, net: Mlp | None, nsteps: int, every: int = 50) -> None:
+    """The network's HoT in place of the DVM's for the next nsteps synthetic
+    solves of ``solver`` (gsis_set_closure_warmup)."""
+    lib = solver.lib
+    capi.check(lib, lib.gsis_set_closure_warmup(solver.h, None if net is None else net.h, int(nsteps), int(every)))
+    solver._closure_warm_net = net
+
+
 def stats(solver) -> dict:
     lib = solver.lib
     f, a, r, x = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()

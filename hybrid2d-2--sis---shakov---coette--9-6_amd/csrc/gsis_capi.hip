@@ -70,6 +70,8 @@ struct gsis_handle {
   gsis_mlp *net = nullptr;
   float cl_gate = 0.0f;
   int cl_every = 0;
+  gsis_mlp *warm_net = nullptr;  // gsis_set_closure_warmup: network and steps left
+  int warm_steps = 0, warm_every = 0;
   gsis::ClosureBufs cl{};
   // MassCorrect (gsis_set_mass_correct): on/off, Init_mass, out_mass on the device
   bool mass_on = false;
@@ -234,22 +236,26 @@ void kinetic(gsis_handle *h) {
 // loop from its starting state without the network.
 constexpr double kClosureGrow = 1.5;
 
-int synthetic_with_closure(gsis_handle *h) {
-  gsis::MlpDev &net = h->net->d;
+int synthetic_with_closure(gsis_handle *h, gsis_mlp *m, float gate, int every, bool replace) {
+  gsis::MlpDev &net = m->d;
   int rc = gsis::synthetic_begin(h->d, h->syn, h->s);
   if (rc) return fail(rc, gsis::synthetic_error());
-  if (gsis::closure_begin(h->d, h->cl, net, h->cl_gate, h->s)) return fail(GSIS_EHIP, "closure: start failed");
+  if (gsis::closure_begin(h->d, h->cl, net, gate, replace, h->s)) return fail(GSIS_EHIP, "closure: start failed");
   bool on = true;
   double res_prev = 0.0;
   for (int k0 = 1;;) {
-    rc = gsis::synthetic_run(h->d, h->syn, k0, on ? k0 + h->cl_every - 1 : 0x7fffffff, h->s);
+    rc = gsis::synthetic_run(h->d, h->syn, k0, on ? k0 + every - 1 : 0x7fffffff, h->s);
     if (rc) return fail(rc, gsis::synthetic_error());
     bool more = false, bad = false;
     int it = 0;
     double res = 0.0;
     rc = gsis::synthetic_state(h->syn, h->s, &more, &it, &bad, &res);
     if (rc) return fail(rc, gsis::synthetic_error());
-    if (on && h->cl.refreshes > 0 && (bad || !(res <= kClosureGrow * res_prev))) {
+    // the residual check, once the network's HoT are in use (replace: from
+    // the first launch on); growth is judged against the launch before
+    const bool net_used = replace || h->cl.refreshes > 0;
+    const bool grew = k0 > 1 && !(res <= kClosureGrow * res_prev);
+    if (on && net_used && (bad || grew || res != res)) {
       if (gsis::closure_reject(h->d, h->cl, h->s)) return fail(GSIS_EHIP, "closure: fallback failed");
       on = false;
       if (it >= h->syn.maxit) break;
@@ -266,6 +272,18 @@ int synthetic_with_closure(gsis_handle *h) {
   return 0;
 }
 
+// the synthetic solve of one step: with the network's HoT in place of the
+// DVM's during a warm-up, with the closure's updates, or plain
+int synthetic_any(gsis_handle *h) {
+  if (h->warm_net && h->warm_steps > 0) {
+    h->warm_steps -= 1;
+    return synthetic_with_closure(h, h->warm_net, 0.0f, h->warm_every, true);
+  }
+  if (h->net) return synthetic_with_closure(h, h->net, h->cl_gate, h->cl_every, false);
+  const int rc = gsis::synthetic_step(h->d, h->syn, h->s);
+  return rc ? fail(rc, gsis::synthetic_error()) : 0;
+}
+
 template <typename T>
 int one_step(gsis_handle *h) {
   gsis::launch_save_s0(h->d, h->s);
@@ -278,7 +296,7 @@ int one_step(gsis_handle *h) {
   if (h->p.if_viscous) gsis::launch_hot(h->d, h->s);
   if (h->p.mode == GSIS_MODE_M1) {
     if (h->timing) (void)hipEventRecord(h->ev[6], h->s);
-    int rc = h->net ? synthetic_with_closure(h) : gsis::synthetic_step(h->d, h->syn, h->s);
+    int rc = synthetic_any(h);
     if (rc) return rc;
     if (h->timing) (void)hipEventRecord(h->ev[7], h->s);
   }
@@ -507,8 +525,8 @@ int batch_replay(gsis_batch *b, int nsteps) {
     HIPCHK(hipStreamSynchronize(b->s));
     for (gsis_handle *h : b->hs) {
       if (h->p.mode != GSIS_MODE_M1) continue;
-      int rc = h->net ? synthetic_with_closure(h) : gsis::synthetic_step(h->d, h->syn, h->s);
-      if (rc) return h->net ? rc : fail(rc, gsis::synthetic_error());
+      int rc = synthetic_any(h);
+      if (rc) return rc;
     }
     for (gsis_handle *h : b->hs) HIPCHK(hipStreamSynchronize(h->s));
   }
@@ -776,6 +794,22 @@ int gsis_set_closure(gsis_handle *h, gsis_mlp *net, double gate, int32_t every) 
   h->net = net;
   h->cl_gate = (float)gate;
   h->cl_every = every;
+  return 0;
+}
+
+int gsis_set_closure_warmup(gsis_handle *h, gsis_mlp *net, int32_t nsteps, int32_t every) {
+  if (!h) return fail(GSIS_EARG, "NULL handle");
+  if (!net || nsteps <= 0) {
+    h->warm_net = nullptr;
+    h->warm_steps = 0;
+    return 0;
+  }
+  if (h->p.mode != GSIS_MODE_M1) return fail(GSIS_EARG, "the closure needs the synthetic solve (mode M1)");
+  if (every < 1) return fail(GSIS_EARG, "closure: every >= 1 required");
+  if (gsis::closure_alloc(h->d, h->cl)) return fail(GSIS_EHIP, "closure: device allocation failed");
+  h->warm_net = net;
+  h->warm_steps = nsteps;
+  h->warm_every = every;
   return 0;
 }
 

@@ -20,6 +20,7 @@ struct ClosureBufs {
   double *usave = nullptr;   // [4][ncell]: U at the start of the inner loop
   int refreshes = 0;         // HoT updates of the last step
   int rejected = 0;          // the last step fell back to the DVM HoT
+  bool replace = false;      // the network's HoT replace the DVM HoT (warm-up) instead of updating them
   void *mem = nullptr;
   size_t bytes = 0;
 };
@@ -36,8 +37,9 @@ void launch_cl_features(const Dev &d, int nfi, int nf, bool from_prim, float *x,
 void launch_cl_targets(const Dev &d, int nfi, int nf, float *y, hipStream_t s);
 // start of a synthetic solve: keep the DVM HoT and the starting state U,
 // evaluate the network at the Convert_NS state, decide which faces use the
-// closure
-int closure_begin(const Dev &d, ClosureBufs &b, MlpDev &net, float gate, hipStream_t s);
+// closure (with replace, every face with a finite prediction, whose HoT are
+// set to the prediction at once)
+int closure_begin(const Dev &d, ClosureBufs &b, MlpDev &net, float gate, bool replace, hipStream_t s);
 // between inner-loop chunks: HoT of the accepted faces from the current state
 int closure_refresh(const Dev &d, ClosureBufs &b, MlpDev &net, hipStream_t s);
 // end of the synthetic solve: the DVM HoT back in place

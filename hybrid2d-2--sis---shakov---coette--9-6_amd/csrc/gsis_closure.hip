@@ -117,9 +117,10 @@ __global__ void k_cl_gate(int nf, const float *ybase, const float *t, const floa
   if ((threadIdx.x & 63) == 0 && b) atomicAdd(count, (int)__popcll(b));
 }
 
-// HoT of the accepted faces = saved DVM HoT + N(now) - N(base)
+// HoT of the accepted faces = saved DVM HoT + N(now) - N(base), or with
+// replace = N(now) itself (tau_33 = -(tau_11 + tau_22))
 __global__ void k_cl_apply(Dev d, int nfi, int nf, const float *ynow, const float *ybase, const int *accept,
-                           const double *saved) {
+                           const double *saved, int replace) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf || !accept[f]) return;
   int dir, i, j;
@@ -130,6 +131,16 @@ __global__ void k_cl_apply(Dev d, int nfi, int nf, const float *ynow, const floa
   const double *st = saved + (dir == 0 ? 0 : 4 * (size_t)N);
   const double *sq = saved + 8 * (size_t)N + (dir == 0 ? 0 : 2 * (size_t)N);
   double dl[kMlpOut];
+  if (replace) {
+    for (int k = 0; k < kMlpOut; k++) dl[k] = (double)ynow[(size_t)f * kMlpOut + k];
+    tg[c] = dl[0];
+    tg[N + c] = dl[1];
+    tg[2 * N + c] = dl[2];
+    tg[3 * N + c] = -dl[0] - dl[1];
+    qg[c] = dl[3];
+    qg[N + c] = dl[4];
+    return;
+  }
   for (int k = 0; k < kMlpOut; k++)
     dl[k] = (double)ynow[(size_t)f * kMlpOut + k] - (double)ybase[(size_t)f * kMlpOut + k];
   tg[c] = st[c] + dl[0];
@@ -206,7 +217,7 @@ void launch_cl_targets(const Dev &d, int nfi, int nf, float *y, hipStream_t s) {
   if (nf > 0) hipLaunchKernelGGL(k_cl_targets, dim3(blocks(nf)), dim3(256), 0, s, d, nfi, nf, y);
 }
 
-int closure_begin(const Dev &d, ClosureBufs &b, MlpDev &net, float gate, hipStream_t s) {
+int closure_begin(const Dev &d, ClosureBufs &b, MlpDev &net, float gate, bool replace, hipStream_t s) {
   if (copy_hot(d, b.saved, true, s) != hipSuccess ||
       hipMemcpyAsync(b.usave, d.U, 4 * (size_t)d.ncell * 8, hipMemcpyDeviceToDevice, s) != hipSuccess)
     return GSIS_EHIP;
@@ -217,8 +228,13 @@ int closure_begin(const Dev &d, ClosureBufs &b, MlpDev &net, float gate, hipStre
   mlp_forward_dev(net, b.x, b.nf, b.ybase, s);
   launch_cl_targets(d, b.nfi, b.nf, b.t, s);
   hipLaunchKernelGGL(k_cl_zero, dim3(1), dim3(64), 0, s, b.count);
-  hipLaunchKernelGGL(k_cl_gate, dim3(blocks(b.nf)), dim3(256), 0, s, b.nf, b.ybase, b.t, net.dscale, gate, b.accept,
-                     b.count);
+  // replacing: every face whose prediction is finite
+  hipLaunchKernelGGL(k_cl_gate, dim3(blocks(b.nf)), dim3(256), 0, s, b.nf, b.ybase, b.t, net.dscale,
+                     replace ? 3.0e38f : gate, b.accept, b.count);
+  b.replace = replace;
+  if (replace)  // the first launch already runs with the network's HoT
+    hipLaunchKernelGGL(k_cl_apply, dim3(blocks(b.nf)), dim3(256), 0, s, d, b.nfi, b.nf, b.ybase, b.ybase, b.accept,
+                       b.saved, 1);
   return hipGetLastError() == hipSuccess ? 0 : GSIS_EHIP;
 }
 
@@ -228,7 +244,7 @@ int closure_refresh(const Dev &d, ClosureBufs &b, MlpDev &net, hipStream_t s) {
   launch_cl_features(d, b.nfi, b.nf, false, b.x, s);
   mlp_forward_dev(net, b.x, b.nf, b.y, s);
   hipLaunchKernelGGL(k_cl_apply, dim3(blocks(b.nf)), dim3(256), 0, s, d, b.nfi, b.nf, b.y, b.ybase, b.accept,
-                     b.saved);
+                     b.saved, b.replace ? 1 : 0);
   return hipGetLastError() == hipSuccess ? 0 : GSIS_EHIP;
 }
 

@@ -112,6 +112,11 @@ module gsis_capi
       real(c_double), value :: gate
       integer(c_int32_t), value :: every
     end function
+    integer(c_int) function gsis_set_closure_warmup(h, net, nsteps, every) bind(C, name="gsis_set_closure_warmup")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h, net
+      integer(c_int32_t), value :: nsteps, every
+    end function
     integer(c_int) function gsis_mlp_create(net) bind(C, name="gsis_mlp_create")
       import :: c_int, c_ptr
       type(c_ptr), intent(out) :: net

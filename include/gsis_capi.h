@@ -288,6 +288,14 @@ int gsis_closure_samples(gsis_handle *h, float *x, float *y, int64_t cap, int64_
  * closure off.  The network must outlive its use and is not used
  * concurrently by two handles. */
 int gsis_set_closure(gsis_handle *h, gsis_mlp *net, double gate, int32_t every);
+/* Warm start: in the synthetic solve of the next nsteps steps (mode M1) the
+ * HoT of every closure face with a finite prediction are the network's,
+ * N(state now), refreshed every `every` inner iterations, in place of the DVM
+ * HoT (which at the start of a run, from a Maxwellian, only cancel the NS
+ * stresses).  The same residual check as gsis_set_closure falls back to the
+ * DVM HoT.  Later steps are plain (or use gsis_set_closure when set), so the
+ * fixed point is the plain one.  net NULL or nsteps <= 0 cancels. */
+int gsis_set_closure_warmup(gsis_handle *h, gsis_mlp *net, int32_t nsteps, int32_t every);
 /* Closure faces, accepted faces and HoT refreshes of the last step, and
  * whether its residual check fell back to the DVM HoT (1) or not (0). */
 int gsis_closure_stats(gsis_handle *h, int64_t *faces, int64_t *accepted, int32_t *refreshes, int32_t *rejected);

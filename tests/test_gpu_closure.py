@@ -128,3 +128,35 @@ def test_wrong_network_is_rejected_and_step_equals_plain():
         assert np.array_equal(a[key], b[key]), key
     plain.close()
     guarded.close()
+
+
+def test_warmup_uses_the_network_then_the_plain_step():
+    """gsis_set_closure_warmup: the first steps' synthetic solves run with the
+    network's HoT (refreshed between launches), later steps are plain; a
+    network whose predictions are not finite is refused on every face, and
+    those steps equal the plain ones bit for bit."""
+    g = load_pkg()
+    from gsis_amd import closure
+    case = g.cases.cavity_case(16, 24, kn=0.1, lid=0.15, dt=1.0)
+    plain, warm, inf = _m1(case), _m1(case), _m1(case)
+    net = closure.trained_network()
+    closure.set_warmup(warm, net, 2, every=5)
+    p = np.zeros(closure.Mlp.nparams(), np.float32)
+    p[-5:] = 10.0  # output bias: times the huge output scale below, +inf
+    scaling = np.concatenate([np.zeros(8), np.ones(8), np.zeros(5), np.full(5, 3e38)]).astype(np.float32)
+    bad = closure.Mlp(p, scaling)
+    closure.set_warmup(inf, bad, 3, every=5)
+    for k in range(3):
+        for s in (plain, warm, inf):
+            s.step(1)
+        st_w, st_i = closure.stats(warm), closure.stats(inf)
+        print("step", k + 1, "warm-up", st_w, "non-finite", st_i, flush=True)
+        if k < 2:
+            assert st_w["accepted"] > 0.9 * st_w["faces"]
+        assert st_i["accepted"] == 0
+    a, b, c = plain.macro(), warm.macro(), inf.macro()
+    for key in ("s", "u", "taoi", "taoj"):
+        assert np.array_equal(a[key], c[key]), key
+    assert not np.array_equal(a["u"], b["u"])  # the warm-up changed the first steps
+    for s in (plain, warm, inf):
+        s.close()
