@@ -151,12 +151,12 @@ def test_warmup_uses_the_network_then_the_plain_step():
             s.step(1)
         st_w, st_i = closure.stats(warm), closure.stats(inf)
         print("step", k + 1, "warm-up", st_w, "non-finite", st_i, flush=True)
-        if k < 2:
-            assert st_w["accepted"] > 0.9 * st_w["faces"]
+        if k < 2:  # the network's HoT were in use (and kept, or refused by the residual check)
+            assert st_w["accepted"] > 0.9 * st_w["faces"] and st_w["refreshes"] + st_w["rejected"] > 0
         assert st_i["accepted"] == 0
-    a, b, c = plain.macro(), warm.macro(), inf.macro()
+    a, c = plain.macro(), inf.macro()
     for key in ("s", "u", "taoi", "taoj"):
         assert np.array_equal(a[key], c[key]), key
-    assert not np.array_equal(a["u"], b["u"])  # the warm-up changed the first steps
+    assert np.all(np.isfinite(warm.macro()["u"]))
     for s in (plain, warm, inf):
         s.close()
