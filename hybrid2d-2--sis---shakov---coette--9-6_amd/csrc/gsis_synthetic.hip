@@ -417,10 +417,13 @@ struct Carry {
 // Lane j owns row j (j = 3 + thread index); step p visits plane i + j = p.
 // Each step is straight-line code: every lane issues the same loads and
 // stores (a lane without a cell stores to the dump), so the waits for a
-// record loaded three steps earlier do not wait for the later ones.  The
-// loop runs a multiple of three steps; the steps after the last plane have no
-// cell in any row.
-template <bool FWD>
+// record loaded kDepth steps earlier do not wait for the later ones.  The
+// loop runs a multiple of kDepth steps; the steps after the last plane have
+// no cell in any row.
+// kDepth records in flight: each step's record is loaded kDepth steps before
+// it is used (their loads come from the device's coherence point, written by
+// other CUs), which takes (kDepth - 1) * 34 more VGPRs than one record
+template <bool FWD, int kDepth>
 __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh) {
   const int nx = d.nx, ny = d.ny, N = d.ncell, nwr = st.nw;
   const double g = d.gamma;
@@ -428,7 +431,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), nw = blockDim.x >> 6;
   const int j = 3 + tid;
   const int pmin = 6, pmax = nx + ny - 6;
-  const int nstep = pmax - pmin + 1, nstep3 = (nstep + 2) / 3 * 3;
+  const int nstep = pmax - pmin + 1, nstepd = (nstep + kDepth - 1) / kDepth * kDepth;
   if (w >= nwr) return;  // a wavefront without rows
   // the wavefront this one takes the neighbouring row from, and the one it
   // hands its edge row to
@@ -448,10 +451,9 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   Carry c;
   for (int m = 0; m < 4; m++) { c.u[m] = 1.0; c.du[m] = 0.0; }
   c.cbi = c.cbj = c.hsi = c.hsj = 0.0;
-  Rec r0, r1, r2;
-  load_rec<FWD>(blk(0), dsb(0), lane, r0);
-  load_rec<FWD>(blk(1), dsb(1), lane, r1);
-  load_rec<FWD>(blk(2), dsb(2), lane, r2);
+  Rec r[kDepth];
+#pragma unroll
+  for (int q = 0; q < kDepth; q++) load_rec<FWD>(blk(q), dsb(q), lane, r[q]);
   auto step = [&](int s, Rec &x) {
     const int p = pof(s), i = p - j;
     const bool on = act(s);
@@ -523,28 +525,28 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the slot is written
       if (lane == 0) lds_put(prog + w, s + 1);
     }
-    load_rec<FWD>(blk(s + 3), dsb(s + 3), lane, x);
+    load_rec<FWD>(blk(s + kDepth), dsb(s + kDepth), lane, x);
   };
-  for (int s = 0; s < nstep3; s += 3) {
-    step(s, r0);
-    step(s + 1, r1);
-    step(s + 2, r2);
+  for (int s = 0; s < nstepd; s += kDepth) {
+#pragma unroll
+    for (int q = 0; q < kDepth; q++) step(s + q, r[q]);
   }
 }
 
 // The wavefronts of a pass are coupled only through their hand-offs: a
 // wavefront may run up to kRing steps ahead of the one that takes its edge row.
+template <int kDepth>
 __device__ void sweep(const Dev &d, const SyntheticState &st, NsShared &sh) {
   const int tid = threadIdx.x;
   if (tid < kMaxSweepWaves) { sh.prog[tid] = 0; sh.cons[tid] = 0; }
   if (tid == 0) sh.abort = 0;
   barrier_lds();
-  sweep_pass<true>(d, st, sh);  // dU* = (Res + dui + duj)/alfa, planes 6 .. nx+ny-6 (:26-69)
+  sweep_pass<true, kDepth>(d, st, sh);  // dU* = (Res + dui + duj)/alfa, planes 6 .. nx+ny-6 (:26-69)
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this lane's dU* stores are complete
   barrier_lds();
   if (tid < kMaxSweepWaves) { sh.prog[tid] = 0; sh.cons[tid] = 0; }
   barrier_lds();
-  sweep_pass<false>(d, st, sh);  // dU = dU* + (dui + duj)/alfa, planes nx+ny-6 .. 6 (:73-111), then U + dU
+  sweep_pass<false, kDepth>(d, st, sh);  // dU = dU* + (dui + duj)/alfa, planes nx+ny-6 .. 6 (:73-111), then U + dU
   // release: the U stores reach the device's coherence point before the
   // flux workgroups are told to read U (L2 write-back, then vmcnt(0))
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -641,7 +643,7 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
       res_last = res;
       __syncthreads();
       if (bad) break;
-      sweep(d, st, sh);
+      sweep<B <= 256 ? 5 : 3>(d, st, sh);  // deeper prefetch where registers allow
       __syncthreads();
       bool more = res > st.tol && k < st.maxit;
       if (more && k >= st.kstop) {  // the launch ends here; a later one continues at k + 1
