@@ -156,11 +156,17 @@ def bench_batch(pkg, solver, args):
     (4 candidates x 1 step) come before the warm-up steps."""
     batch = _load("gsis_amd.batch", PKG_DIR / "batch.py")
     t0 = time.perf_counter()
-    cases = batch.instances(args.instances, seed=args.seed)
+    m1 = args.mode == "m1"
+    # GSIS instances: first-order upwind (with NND the cavity runs do not converge), MassCorrect, at most 1000 inner
+    # iterations per step; the kinetic step alone: NND
+    cases = batch.instances(args.instances, seed=args.seed, precision_scheme=pkg.capi.SCHEME_UP1 if m1 else None)
     solvers = []
     for c in cases:
-        s = solver.GsisSolver(c, precision=pkg.capi.PREC_FP32, kstep_show=1 << 30)
+        s = solver.GsisSolver(c, precision=pkg.capi.PREC_FP32, kstep_show=1 << 30,
+                              mode=pkg.capi.MODE_M1 if m1 else pkg.capi.MODE_M0, ns_maxit=1000 if m1 else 0)
         s.initialize()
+        if m1:
+            s.set_mass_correct(True)
         solvers.append(s)
     b = batch.Batch(solvers)
     units = sum(c.ncell * c.nv for c in cases)
@@ -174,6 +180,7 @@ def bench_batch(pkg, solver, args):
     t1 = time.perf_counter()
     b.step(args.steps)
     el = time.perf_counter() - t1
+    its = [sv.counters()[2] for sv in solvers] if m1 else None
     ms = el / args.steps * 1e3
     alg = (16 * units + 8 * ring) / (ms * 1e-3) / 1e9  # FP32: read+write G,H once; moments read G,H
     ncou = sum("couette" in c.name for c in cases)
@@ -187,10 +194,13 @@ def bench_batch(pkg, solver, args):
         "config": {"workload": f"batch of {len(cases)} instances from seed {args.seed}: {ncou} planar Couette "
                                f"(4 x side cells) and {len(cases) - ncou} lid-driven cavities (side^2 cells), side "
                                "64-256, Kn log-uniform 1e-3-10, wall/lid speed 0.05-0.3, velocity grid 24^2-64^2, "
-                               "NND fluxes, kinetic step (mode M0), one hipGraph replay per step",
-                   "instances": len(cases), "cell_velocities": units},
+                               + ("GSIS (mode M1: UP1 fluxes, MassCorrect, at most 1000 inner iterations), the inner "
+                                  "loops of all instances in one launch inside the step's graph" if m1 else
+                                  "NND fluxes, kinetic step (mode M0)") + ", one hipGraph replay per step",
+                   "instances": len(cases), "cell_velocities": units, "mode": args.mode},
         "autotuner": {"columns_per_workgroup": nw, "order": "largest first" if order else "as drawn",
                       "ms_per_step": tune_ms},
+        "inner_iterations_last_step": {"max": max(its), "mean": sum(its) / len(its)} if its else None,
         "roofline": {"bound": "hbm", "kernel": "batched step", "achieved": alg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / HBM_PEAK_GBS, "traffic": None,
                      "note": "algorithmic bytes of the batched kinetic launch and the moments over the step time"},

@@ -410,6 +410,8 @@ struct gsis_batch {
   int *epoch = nullptr;   // [0] launch epoch, [1] a seam wait timed out
   std::vector<int *> flags;
   int nstrip = 0, nseam = 0, ncellb = 0;
+  gsis::NsJob *nsjobs = nullptr;  // device table of the inner loops run in the graph
+  std::vector<bool> in_graph;     // M1 instance whose synthetic solve is captured
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<void *> allocs;
@@ -476,6 +478,27 @@ int batch_capture(gsis_batch *b) {
   batch_drop_graph(b);
   int rc = batch_jobs<T>(b);
   if (rc) return rc;
+  {  // which M1 instances run their inner loop inside the graph
+    std::vector<gsis::NsJob> jobs;
+    b->in_graph.assign(b->hs.size(), false);
+    for (size_t k = 0; k < b->hs.size(); k++) {
+      gsis_handle *h = b->hs[k];
+      if (h->p.mode != GSIS_MODE_M1 || h->net || (h->warm_net && h->warm_steps > 0) ||
+          !gsis::synthetic_batchable(h->syn))
+        continue;
+      b->in_graph[k] = true;
+      jobs.push_back(gsis::NsJob{h->d, h->syn});
+    }
+    if (!jobs.empty()) {
+      if (!b->nsjobs) {
+        void *q = nullptr;
+        HIPCHK(hipMalloc(&q, b->hs.size() * sizeof(gsis::NsJob)));
+        b->allocs.push_back(q);
+        b->nsjobs = reinterpret_cast<gsis::NsJob *>(q);
+      }
+      HIPCHK(hipMemcpy(b->nsjobs, jobs.data(), jobs.size() * sizeof(gsis::NsJob), hipMemcpyHostToDevice));
+    }
+  }
   std::vector<hipStream_t> own(b->hs.size());
   for (size_t k = 0; k < b->hs.size(); k++) {
     own[k] = b->hs[k]->s;
@@ -495,6 +518,15 @@ int batch_capture(gsis_batch *b) {
       gsis::launch_convert(h->d, b->s);
       if (h->p.if_viscous) gsis::launch_hot(h->d, b->s);
     }
+    // the synthetic solves of the M1 instances whose inner loop fits one
+    // workgroup and that use no closure: all in one launch, one workgroup each
+    int nj = 0;
+    for (size_t k = 0; k < b->hs.size(); k++)
+      if (b->in_graph[k]) gsis::synthetic_begin(b->hs[k]->d, b->hs[k]->syn, b->s);
+    for (size_t k = 0; k < b->hs.size(); k++) nj += b->in_graph[k] ? 1 : 0;
+    gsis::launch_ns_batch(b->nsjobs, nj, b->s);
+    for (size_t k = 0; k < b->hs.size(); k++)
+      if (b->in_graph[k]) gsis::synthetic_finish(b->hs[k]->d, b->hs[k]->syn, b->s);
     const hipError_t e2 = hipStreamEndCapture(b->s, &b->graph);
     e = e2 != hipSuccess ? e2 : hipGetLastError();
   }
@@ -512,19 +544,20 @@ int batch_capture_any(gsis_batch *b) {
 }
 
 int batch_replay(gsis_batch *b, int nsteps) {
-  bool m1 = false;
-  for (gsis_handle *h : b->hs) {
-    HIPCHK(hipStreamSynchronize(h->s));
-    m1 = m1 || h->p.mode == GSIS_MODE_M1;
+  bool m1 = false;  // an M1 instance whose synthetic solve runs outside the graph
+  for (size_t q = 0; q < b->hs.size(); q++) {
+    HIPCHK(hipStreamSynchronize(b->hs[q]->s));
+    m1 = m1 || (b->hs[q]->p.mode == GSIS_MODE_M1 && !b->in_graph[q]);
   }
   for (int k = 0; k < nsteps; k++) {
     HIPCHK(hipGraphLaunch(b->exec, b->s));
     if (!m1) continue;
-    // the synthetic solve of each M1 instance after the captured part of its
-    // step (a cooperative launch and host-side loop control: not captured)
+    // the synthetic solve of each other M1 instance after the captured part
+    // of its step (a cooperative launch, or the closure's host-side loop)
     HIPCHK(hipStreamSynchronize(b->s));
-    for (gsis_handle *h : b->hs) {
-      if (h->p.mode != GSIS_MODE_M1) continue;
+    for (size_t q = 0; q < b->hs.size(); q++) {
+      gsis_handle *h = b->hs[q];
+      if (h->p.mode != GSIS_MODE_M1 || b->in_graph[q]) continue;
       int rc = synthetic_any(h);
       if (rc) return rc;
     }

@@ -64,6 +64,17 @@ void synthetic_finish(const Dev &d, SyntheticState &st, hipStream_t s);
 // iteration (synchronises the stream)
 int synthetic_state(SyntheticState &st, hipStream_t s, bool *unfinished, int *iters, bool *bad_t, double *res);
 int synthetic_read_iters(SyntheticState &st, hipStream_t s, int *iters, int *bad_t);
+// One instance's inner loop in a batched launch (k_ns_batch)
+struct NsJob {
+  Dev d;
+  SyntheticState st;
+};
+// whether an instance's inner loop fits one 256-thread workgroup (at most
+// 256 swept rows)
+bool synthetic_batchable(const SyntheticState &st);
+// the inner loops of n instances (device table djobs), one workgroup each,
+// between their synthetic_begin and synthetic_finish
+void launch_ns_batch(const NsJob *djobs, int n, hipStream_t s);
 void synthetic_free(SyntheticState &st);
 const char *synthetic_error();
 

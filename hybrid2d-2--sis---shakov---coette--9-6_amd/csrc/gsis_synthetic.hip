@@ -663,6 +663,53 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
   }
 }
 
+// The inner loops of a batch of instances in one launch, one workgroup per
+// instance: the workgroup computes the fluxes of its instance's tiles one
+// after another, then runs the same sweeps, until the reference's rule ends
+// the loop.  No workgroup waits for another, so the launch needs neither
+// co-residency nor host decisions and can be captured in a graph.  The same
+// records and arithmetic as k_ns_loop: the same dU and iteration counts.
+__global__ void __launch_bounds__(256) k_ns_batch(const NsJob *__restrict__ jobs) {
+  __shared__ NsShared sh;
+  const NsJob &J = jobs[blockIdx.x];
+  const Dev &d = J.d;
+  const SyntheticState &st = J.st;
+  const int tid = threadIdx.x;
+  int it = 0, bad = 0;
+  double res_last = 0.0;
+  const bool go = 1.0 > st.tol && st.maxit > 0;  // the reference starts from res = 1
+  if (go && st.rs > 0) {
+    const int ntiles = st.ntx * st.nty;
+    for (int k = 1;; k++) {
+      double rmax = 0.0;
+      int b = 0;
+      for (int tile = 0; tile < ntiles; tile++) flux_tile(d, st, sh, tile, rmax, b);
+      for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_down(rmax, off, 64));
+      b = __syncthreads_or(b);
+      if ((tid & 63) == 0) sh.red[tid >> 6] = rmax;
+      __syncthreads();
+      double res = sh.red[0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); q++) res = fmax(res, sh.red[q]);
+      it = k;
+      res_last = res;
+      bad = b;
+      __syncthreads();
+      if (bad) break;
+      sweep<5>(d, st, sh);
+      __syncthreads();
+      if (!(res > st.tol && k < st.maxit)) break;
+    }
+  } else if (go) {
+    it = 1;  // no swept cell: one iteration that changes nothing
+  }
+  if (tid == 0) {
+    st.iters[0] = it;
+    st.iters[1] = bad;
+    st.iters[2] = 0;
+    reinterpret_cast<double *>(st.iters + 4)[0] = res_last;
+  }
+}
+
 // U0 <- U and the synthetic correction of S on cells 3..n-3 (src/Hybrid_Sub.f90:108-153)
 __global__ void k_ns_feedback(Dev d, SyntheticState st) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -796,6 +843,12 @@ int synthetic_run(const Dev &d, SyntheticState &st, int k0, int kstop, hipStream
     hipLaunchKernelGGL(k_ns_empty, dim3(1), dim3(64), 0, s, st);
   }
   return 0;
+}
+
+bool synthetic_batchable(const SyntheticState &st) { return st.on && st.block <= 256; }
+
+void launch_ns_batch(const NsJob *djobs, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_ns_batch, dim3(n), dim3(256), 0, s, djobs);
 }
 
 void synthetic_finish(const Dev &d, SyntheticState &st, hipStream_t s) {
