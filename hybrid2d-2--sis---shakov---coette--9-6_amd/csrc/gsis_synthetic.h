@@ -8,20 +8,20 @@ namespace gsis {
 // Per-cell inputs of the LU-SGS sweeps of the cells i, j in [3, n-3], written
 // by the flux workgroups of the inner loop: field k of each cell.  Stored by
 // anti-diagonal plane p = i+j (the sweep order) and by the 64 rows of a sweep
-// wavefront: block ((p-6)*nw + (j-3)/64) holds field k of row j at
-// k*64 + (j-3)%64, so a wavefront reads each field of a plane with one
-// contiguous load.
+// wavefront, two fields to a 16-byte pair: block ((p-6)*nw + (j-3)/64) holds
+// field k of row j at (k/2)*128 + 2*((j-3)%64) + k%2, so a wavefront reads two
+// fields of a plane with one contiguous 16-byte-per-lane load.
 enum NsRec {
-  kRecU = 0,     // U(1:4) before this iteration's update
-  kRecCbi = 4,   // Lci/2 + Lvi: the weight of dU in the i-direction off-diagonal terms
-  kRecCbj = 5,   // Lcj/2 + Lvj
-  kRecHsi = 6,   // si(i,j)/2
-  kRecHsj = 7,   // sj(i,j)/2
+  kRecRes = 0,   // Res(1:4)
+  kRecU = 4,     // U(1:4) before this iteration's update
   kRecRa = 8,    // 1/alfa, the inverse diagonal
-  kRecRes = 9,   // Res(1:4)
+  kRecCbi = 9,   // Lci/2 + Lvi: the weight of dU in the i-direction off-diagonal terms
+  kRecCbj = 10,  // Lcj/2 + Lvj
+  kRecHsi = 11,  // si(i,j)/2
+  kRecHsj = 12,  // sj(i,j)/2
   kRecNi = 13,   // ni1(i,j), ni2(i,j)
   kRecNj = 15,   // nj1(i,j), nj2(i,j)
-  kNRec = 17
+  kNRec = 18     // 17 fields and one unused: nine pairs
 };
 
 struct SyntheticState {
@@ -36,10 +36,10 @@ struct SyntheticState {
   int nflux = 0;             // flux workgroups of the inner-loop launch
   int block = 256;           // threads per workgroup of the inner-loop launch
   double *s00 = nullptr;     // [4][ncell]  S00_GKUA(1:4)
-  double *rec = nullptr;     // [np][nw][kNRec][64] sweep inputs, see NsRec
-  double *dstar = nullptr;   // [np][nw][4][64] dU after the forward sweep
+  double *rec = nullptr;     // [np][nw][kNRec/2][64][2] sweep inputs, see NsRec
+  double *dstar = nullptr;   // [np][nw][2][64][2] dU after the forward sweep (components in pairs)
   double *part = nullptr;    // [nflux][8]: per flux workgroup {iteration done (int), T<0.1 (int), max |Res(4)|}
-  double *dump = nullptr;    // [4][64] sink of the sweep's stores for lanes without a cell
+  double *dump = nullptr;    // [256] sink of the sweep's stores for lanes without a cell
   int *ctl = nullptr;        // [1] go (next iteration or -1), [3] a wait timed out
   int *iters = nullptr;      // [0]: inner iterations of the last step, [1]: T<0.1 flag,
                              // [2]: the launch stopped at kstop with the loop unfinished,

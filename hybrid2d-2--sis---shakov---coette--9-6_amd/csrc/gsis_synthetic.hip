@@ -55,14 +55,36 @@ constexpr long long kWaitTicks = 200000000;  // bounded waits: 2 s of the 100 MH
 enum { kCtlGo = 1, kCtlErr = 3 };
 
 // Sweep records are blocked by (plane, wavefront): block ((p-6)*nw + w) holds
-// field k of the 64 rows 3+64w .. 3+64w+63 at k*64 + lane, so each field of a
-// plane is one contiguous 512 B load per wavefront.
-__device__ __forceinline__ size_t rec_blk(int p, int w, int nw) { return ((size_t)(p - 6) * nw + w) * (kNRec * 64); }
-__device__ __forceinline__ size_t rec_at(int i, int j, int k, int nw) {
+// the 64 rows 3+64w .. 3+64w+63, field k of row 3+64w+l at (k/2)*128 + 2l + k%2,
+// so each pair of fields of a plane is one contiguous 1 KiB load per
+// wavefront.  Offsets in doubles; the record array stays below 4 GiB
+// (synthetic_init), so byte offsets fit the 32-bit buffer offsets.
+constexpr int kBlk = kNRec * 64;  // doubles per block
+static_assert(kRecRes == 0 && kRecU == 4 && kRecRa == 8 && kRecCbi == 9 && kRecCbj == 10 && kRecHsi == 11 &&
+                  kRecHsj == 12 && kRecNi == 13 && kRecNj == 15 && kNRec == 18,
+              "the sweep's pair loads and the flux workgroups' pair stores assume this field order");
+__device__ __forceinline__ size_t rec_blk(int p, int w, int nw) { return ((size_t)(p - 6) * nw + w) * kBlk; }
+__device__ __forceinline__ size_t rec_cell(int i, int j, int nw) {  // the cell's first pair
   const int r = j - 3;
-  return rec_blk(i + j, r >> 6, nw) + k * 64 + (r & 63);
+  return rec_blk(i + j, r >> 6, nw) + 2 * (r & 63);
 }
-__device__ __forceinline__ size_t dstar_at(int p, int w, int nw) { return ((size_t)(p - 6) * nw + w) * 256; }
+
+// The record array as a raw buffer; loads and stores through it carry sc1,
+// the cache policy of ld_agent / st_agent, so they are coherent across the
+// device's XCDs.
+constexpr int kBufWord3 = 0x00020000;  // gfx9-family descriptor word 3 (raw buffer, stride 0)
+constexpr int kCoherent = 16;          // buffer instruction cache policy: sc1
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const SyntheticState &st) {
+  return __builtin_amdgcn_make_buffer_rsrc(st.rec, (short)0, (int)((size_t)st.np * st.nw * kBlk * sizeof(double)),
+                                           kBufWord3);
+}
+__device__ __forceinline__ void st_pair(__amdgpu_buffer_rsrc_t r, unsigned byte_off, double a, double b) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, double2{a, b}), r, byte_off, 0, kCoherent);
+}
+__device__ __forceinline__ double2 ld_pair(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, kCoherent));
+}
 
 // S0 <- S - S0, S00 <- S on interior cells; the kinetic increment added to the
 // synthetic state of the previous step on cells 3..n-3 (src/Hybrid_Sub.f90:45-94).
@@ -288,7 +310,7 @@ __device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, 
   }
   __syncthreads();
   const int nw = st.nw;
-  double *R = st.rec;
+  const __amdgpu_buffer_rsrc_t rr = rec_rsrc(st);
   for (int t = tid; t < nci * ncj; t += nt) {
     const int a = t % nci, b = t / nci, i = i0 + a, j = j0 + b;
     double res[4];
@@ -319,39 +341,34 @@ __device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, 
     }
     double alfa = d.vol[g0] / dt + 1.0 * (lci + lcj);
     if (visc) alfa = alfa + 2.0 * (lvi + lvj);
-    double uc[4];
-    for (int m = 0; m < 4; m++) {
-      uc[m] = sh.U[m][lc];
-      st_agent(R + rec_at(i, j, kRecU + m, nw), uc[m]);
-      st_agent(R + rec_at(i, j, kRecRes + m, nw), res[m]);
-    }
-    st_agent(R + rec_at(i, j, kRecCbi, nw), 0.5 * lci + lvi);
-    st_agent(R + rec_at(i, j, kRecCbj, nw), 0.5 * lcj + lvj);
-    st_agent(R + rec_at(i, j, kRecHsi, nw), 0.5 * d.si[g0]);
-    st_agent(R + rec_at(i, j, kRecHsj, nw), 0.5 * d.sj[g0]);
-    st_agent(R + rec_at(i, j, kRecRa, nw), 1.0 / alfa);
-    const double n1i = d.ni1[g0], n2i = d.ni2[g0], n1j = d.nj1[g0], n2j = d.nj2[g0];
-    st_agent(R + rec_at(i, j, kRecNi, nw), n1i);
-    st_agent(R + rec_at(i, j, kRecNi + 1, nw), n2i);
-    st_agent(R + rec_at(i, j, kRecNj, nw), n1j);
-    st_agent(R + rec_at(i, j, kRecNj + 1, nw), n2j);
+    const unsigned o = (unsigned)(rec_cell(i, j, nw) * sizeof(double));
+    st_pair(rr, o + (kRecRes / 2) * 1024, res[0], res[1]);
+    st_pair(rr, o + (kRecRes / 2 + 1) * 1024, res[2], res[3]);
+    st_pair(rr, o + (kRecU / 2) * 1024, sh.U[0][lc], sh.U[1][lc]);
+    st_pair(rr, o + (kRecU / 2 + 1) * 1024, sh.U[2][lc], sh.U[3][lc]);
+    st_pair(rr, o + (kRecRa / 2) * 1024, 1.0 / alfa, 0.5 * lci + lvi);                // ra, cbi
+    st_pair(rr, o + (kRecCbj / 2) * 1024, 0.5 * lcj + lvj, 0.5 * d.si[g0]);           // cbj, hsi
+    st_pair(rr, o + (kRecHsj / 2) * 1024, 0.5 * d.sj[g0], d.ni1[g0]);                 // hsj, ni1
+    st_pair(rr, o + (kRecNi / 2 + 1) * 1024, d.ni2[g0], d.nj1[g0]);                   // ni2, nj1
+    st_pair(rr, o + (kRecNj / 2 + 1) * 1024, d.nj2[g0], 0.0);                         // nj2
   }
   __syncthreads();  // the next tile reuses the LDS
 }
 
 __device__ __forceinline__ void barrier_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// x of lane l-1 (wave_shr:1) or l+1 (wave_shl:1) of the wavefront; 0 in the
-// lane that has no such neighbour
-__device__ __forceinline__ double lane_shift(double x, bool up) {
-  const unsigned long long b = __double_as_longlong(x);
+// x of lane l-1 (wave_shr:1) or l+1 (wave_shl:1) of the wavefront; edge in
+// the lane that has no such neighbour (lane 0, or lane 63)
+__device__ __forceinline__ double lane_shift(double x, bool up, double edge) {
+  const unsigned long long b = __double_as_longlong(x), e = __double_as_longlong(edge);
   int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+  const int elo = (int)(unsigned)e, ehi = (int)(unsigned)(e >> 32);
   if (up) {
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, false);
+    lo = __builtin_amdgcn_update_dpp(elo, lo, 0x138, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(ehi, hi, 0x138, 0xf, 0xf, false);
   } else {
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, false);
+    lo = __builtin_amdgcn_update_dpp(elo, lo, 0x130, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(ehi, hi, 0x130, 0xf, 0xf, false);
   }
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
@@ -363,25 +380,30 @@ struct Rec {
 };
 
 // Every lane loads its row's slot of the plane, whether or not the row has a
-// cell there (the slot exists for every row of a sweep wavefront), so the
-// loads take one base address and immediate offsets.
+// cell there (the slot exists for every row of a sweep wavefront): nine
+// 16-byte loads, one scalar block offset and immediate pair offsets.
 template <bool FWD>
-__device__ __forceinline__ void load_rec(const double *bp, const double *ds, int lane, Rec &x) {
-  const double *b = bp + lane;
-  const double *e = ds + lane;
-  for (int m = 0; m < 4; m++) {
-    x.res[m] = FWD ? ld_agent(b + (kRecRes + m) * 64) : e[m * 64];  // dU* was stored by this lane
-    x.u[m] = ld_agent(b + (kRecU + m) * 64);
+__device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t rr, int blk_bytes, const double2 *ds, int lane,
+                                         Rec &x) {
+  const unsigned v = (unsigned)lane * 16u;
+  auto pr = [&](int q) { return ld_pair(rr, v + (unsigned)q * 1024u, blk_bytes); };
+  double2 r01, r23;
+  if (FWD) {
+    r01 = pr(kRecRes / 2);
+    r23 = pr(kRecRes / 2 + 1);
+  } else {  // dU* was stored by this lane
+    r01 = ds[0];
+    r23 = ds[64];
   }
-  x.ra = ld_agent(b + kRecRa * 64);
-  for (int m = 0; m < 2; m++) {
-    x.ni[m] = ld_agent(b + (kRecNi + m) * 64);
-    x.nj[m] = ld_agent(b + (kRecNj + m) * 64);
-  }
-  x.hsi = ld_agent(b + kRecHsi * 64);
-  x.hsj = ld_agent(b + kRecHsj * 64);
-  x.cbi = ld_agent(b + kRecCbi * 64);
-  x.cbj = ld_agent(b + kRecCbj * 64);
+  const double2 u01 = pr(kRecU / 2), u23 = pr(kRecU / 2 + 1), p4 = pr(4), p5 = pr(5), p6 = pr(6), p7 = pr(7),
+                p8 = pr(8);
+  x.res[0] = r01.x; x.res[1] = r01.y; x.res[2] = r23.x; x.res[3] = r23.y;
+  x.u[0] = u01.x; x.u[1] = u01.y; x.u[2] = u23.x; x.u[3] = u23.y;
+  x.ra = p4.x; x.cbi = p4.y;
+  x.cbj = p5.x; x.hsi = p5.y;
+  x.hsj = p6.x; x.ni[0] = p6.y;
+  x.ni[1] = p7.x; x.nj[0] = p7.y;
+  x.nj[1] = p8.x;
 }
 
 // LDS words read and written by several wavefronts of the workgroup
@@ -389,6 +411,8 @@ __device__ __forceinline__ int lds_get(int *p) { return __hip_atomic_load(p, __A
 __device__ __forceinline__ void lds_put(int *p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+__device__ __forceinline__ void lgkm_wait() { __builtin_amdgcn_s_waitcnt(0xC07F); }  // lgkmcnt(0)
+__device__ __forceinline__ void mem_order() { asm volatile("" ::: "memory"); }  // no compiler reordering
 
 // Waits until the LDS word *p reaches want; bounded by kWaitTicks, after
 // which (or once another wavefront has timed out) it gives up, sets the
@@ -422,13 +446,29 @@ struct Carry {
 // no cell in any row.
 // kDepth records in flight: each step's record is loaded kDepth steps before
 // it is used (their loads come from the device's coherence point, written by
-// other CUs), which takes (kDepth - 1) * 34 more VGPRs than one record
+// other CUs), which takes (kDepth - 1) * 34 more VGPRs than one record.
+//
+// Hand-offs between wavefronts (LDS ring of kRing slots per wavefront): the
+// edge lane (63 forward, 0 backward) of wavefront w writes its values of step
+// s in slot s % kRing; prog[w] = s + 1 says slots up to s are written,
+// cons[w] = s + 1 that wavefront w has taken the slots up to s of its
+// neighbour.  No step waits for an LDS round trip in the steady state:
+//  * the words and the neighbour's next slot are read at the end of a step,
+//    one step before they are needed (first the word, then the slot: the LDS
+//    performs a wavefront's operations in order, so a slot read after a word
+//    that says "written" holds the written values);
+//  * the next step first works on its own i-neighbour, then waits for those
+//    reads (lgkmcnt(0)), publishes prog and cons, and only if the word it read
+//    was too old waits for the neighbour and reads the slot again.  A
+//    wavefront so settles one to two steps behind its neighbour;
+//  * a slot is written only when the word cons read one step earlier says it
+//    has been taken (cons only grows), else after waiting for it.
 template <bool FWD, int kDepth>
 __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh) {
   const int nx = d.nx, ny = d.ny, N = d.ncell, nwr = st.nw;
   const double g = d.gamma;
   const int tid = threadIdx.x, lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), nw = blockDim.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int j = 3 + tid;
   const int pmin = 6, pmax = nx + ny - 6;
   const int nstep = pmax - pmin + 1, nstepd = (nstep + kDepth - 1) / kDepth * kDepth;
@@ -437,50 +477,67 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   // hands its edge row to
   const int from = FWD ? w - 1 : w + 1, to = FWD ? w + 1 : w - 1;
   const bool has_from = from >= 0 && from < nwr, has_to = to >= 0 && to < nwr;
-  int *prog = sh.prog, *cons = sh.cons, *abort = &sh.abort;
-  const double *R = st.rec;
-  double *dstar = st.dstar;
-  double *dump = st.dump + lane;
+  int *prog = sh.prog, *cons = sh.cons, *abort = &sh.abort, *err = st.ctl + kCtlErr;
+  const __amdgpu_buffer_rsrc_t rr = rec_rsrc(st);
+  const double2 *dstar = reinterpret_cast<const double2 *>(st.dstar);
+  double2 *dump2 = reinterpret_cast<double2 *>(st.dump) + lane;  // [2][64] pairs
+  double *dump1 = st.dump + lane;                                  // [4][64]
+  // the plane of step s (clamped to the last one: the steps past it reload it)
+  // as a block index (p - 6) * nw + w
+  auto bidx = [&](int s) {
+    const int q = min(s, nstep - 1);
+    return (FWD ? q : nstep - 1 - q) * nwr + w;
+  };
+  auto blk_bytes = [&](int s) { return bidx(s) * (kBlk * (int)sizeof(double)); };
+  auto dsb = [&](int s) { return dstar + (size_t)bidx(s) * 128 + lane; };
   auto pof = [&](int s) { return FWD ? pmin + s : pmax - s; };
   auto act = [&](int s) {
     const int p = pof(s);
     return s < nstep && j <= ny - 3 && p - j >= 3 && p - j <= nx - 3;
   };
-  auto blk = [&](int s) { return R + rec_blk(pof(min(s, nstep - 1)), w, nwr); };
-  auto dsb = [&](int s) { return dstar + dstar_at(pof(min(s, nstep - 1)), w, nwr); };
+  // the neighbour's slot of step t: U, dU, Cbj, Hsj of its edge row
+  auto read_slot = [&](int t, double e[10]) {
+    const double *xb = sh.xw[t % kRing][from];
+#pragma unroll
+    for (int m = 0; m < 10; m++) e[m] = xb[m];
+  };
   Carry c;
   for (int m = 0; m < 4; m++) { c.u[m] = 1.0; c.du[m] = 0.0; }
   c.cbi = c.cbj = c.hsi = c.hsj = 0.0;
+  double e[10];  // the neighbour's edge row of the previous step (0 without a neighbour)
+#pragma unroll
+  for (int m = 0; m < 10; m++) e[m] = 0.0;
+  int fprog = 0, fcons = 0;  // prog[from] and cons[to] as read one step earlier
   Rec r[kDepth];
 #pragma unroll
-  for (int q = 0; q < kDepth; q++) load_rec<FWD>(blk(q), dsb(q), lane, r[q]);
+  for (int q = 0; q < kDepth; q++) load_rec<FWD>(rr, blk_bytes(q), dsb(q), lane, r[q]);
   auto step = [&](int s, Rec &x) {
     const int p = pof(s), i = p - j;
     const bool on = act(s);
-    // the neighbouring row's values from the previous step: the lane below
-    // (forward) or above (backward), through LDS across wavefronts
-    double nu[4], ncb, nhs, ndu[4];
-    for (int m = 0; m < 4; m++) {
-      nu[m] = lane_shift(c.u[m], FWD);
-      ndu[m] = lane_shift(c.du[m], FWD);
-    }
-    ncb = lane_shift(c.cbj, FWD);
-    nhs = lane_shift(c.hsj, FWD);
-    if (has_from && s > 0) {  // the edge row of step s-1 from the neighbouring wavefront
-      wait_lds(prog + from, s, abort, st.ctl + kCtlErr);
-      if (FWD ? lane == 0 : lane == 63) {
-        const double *xb = sh.xw[(s - 1) % kRing][from];
-        for (int m = 0; m < 4; m++) { nu[m] = xb[m]; ndu[m] = xb[4 + m]; }
-        ncb = xb[8];
-        nhs = xb[9];
-      }
-      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the slot is read
-      if (lane == 0) lds_put(cons + w, s);
-    }
-    // off-diagonal terms from the i-neighbour (this lane's previous cell) and
-    // the j-neighbour
-    double dfi[4], dfj[4], acc[4];
+    // off-diagonal term from the i-neighbour (this lane's previous cell)
+    double dfi[4], dfj[4];
     dfn(c.u, c.du, x.ni[0], x.ni[1], g, dfi);
+    if (s > 0) {
+      lgkm_wait();  // the previous step's slot write and reads are complete
+      if (has_to && lane == 0) lds_put(prog + w, s);
+      if (has_from) {
+        if (__builtin_amdgcn_readfirstlane(fprog) < s) {  // read too early: wait for slot s-1, read it again
+          wait_lds(prog + from, s, abort, err);
+          mem_order();
+          read_slot(s - 1, e);
+          lgkm_wait();
+        }
+        if (lane == 0) lds_put(cons + w, s);
+      }
+    }
+    // the j-neighbour: the lane below (forward) or above (backward), the
+    // neighbouring wavefront's edge row in the edge lane
+    double nu[4], ndu[4];
+    for (int m = 0; m < 4; m++) {
+      nu[m] = lane_shift(c.u[m], FWD, e[m]);
+      ndu[m] = lane_shift(c.du[m], FWD, e[4 + m]);
+    }
+    const double ncb = lane_shift(c.cbj, FWD, e[8]), nhs = lane_shift(c.hsj, FWD, e[9]);
     dfn(nu, ndu, x.nj[0], x.nj[1], g, dfj);
     const bool usei = FWD ? i != 3 : i != nx - 3, usej = FWD ? j != 3 : j != ny - 3;
     double du[4];
@@ -492,17 +549,17 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       double a = FWD ? x.res[m] : 0.0;
       a = usei ? a + ti : a;
       a = usej ? a + tj : a;
-      acc[m] = a;
-      du[m] = FWD ? acc[m] * x.ra : x.res[m] + acc[m] * x.ra;  // backward: x.res holds dU*
+      du[m] = FWD ? a * x.ra : x.res[m] + a * x.ra;  // backward: x.res holds dU*
     }
     // plain stores: dU* is read back by this lane, U is written back to the
     // device's coherence point by the release fence after the sweep
     if (FWD) {
-      double *o = on ? dstar + dstar_at(p, w, nwr) + lane : dump;
-      for (int m = 0; m < 4; m++) o[m * 64] = du[m];
+      double2 *o = on ? reinterpret_cast<double2 *>(st.dstar) + (size_t)bidx(s) * 128 + lane : dump2;
+      o[0] = double2{du[0], du[1]};
+      o[64] = double2{du[2], du[3]};
     } else {
       for (int m = 0; m < 4; m++) {
-        double *o = on ? d.U + m * N + cidx(d, i, j) : dump + m * 64;
+        double *o = on ? d.U + m * N + cidx(d, i, j) : dump1 + m * 64;
         *o = x.u[m] + du[m];
       }
     }
@@ -514,18 +571,27 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     c.cbj = x.cbj;
     c.hsi = x.hsi;
     c.hsj = x.hsj;
-    if (has_to) {  // hand the edge row on; slot s % kRing must have been taken
-      wait_lds(cons + to, s - kRing + 1, abort, st.ctl + kCtlErr);
+    if (has_to) {  // hand the edge row on, once slot s % kRing has been taken
+      if (__builtin_amdgcn_readfirstlane(fcons) < s - kRing + 1) {
+        wait_lds(cons + to, s - kRing + 1, abort, err);
+        mem_order();
+      }
       if (FWD ? lane == 63 : lane == 0) {
         double *xb = sh.xw[s % kRing][w];
         for (int m = 0; m < 4; m++) { xb[m] = c.u[m]; xb[4 + m] = c.du[m]; }
         xb[8] = c.cbj;
         xb[9] = c.hsj;
       }
-      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the slot is written
-      if (lane == 0) lds_put(prog + w, s + 1);
+      mem_order();
+      fcons = lds_get(cons + to);
     }
-    load_rec<FWD>(blk(s + kDepth), dsb(s + kDepth), lane, x);
+    if (has_from) {  // for the next step: the word, then the slot of this step
+      mem_order();
+      fprog = lds_get(prog + from);
+      mem_order();
+      read_slot(s, e);
+    }
+    load_rec<FWD>(rr, blk_bytes(s + kDepth), dsb(s + kDepth), lane, x);
   };
   for (int s = 0; s < nstepd; s += kDepth) {
 #pragma unroll
@@ -782,6 +848,10 @@ int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st) {
   }
   st.nw = (st.rs + 63) / 64;
   const size_t nq = (size_t)st.np * st.nw * 64;  // cells of the blocked sweep layout
+  if ((size_t)kNRec * nq * sizeof(double) >= ((size_t)1 << 31)) {  // 32-bit buffer offsets of the records
+    g_syn_err = "synthetic solve: the mesh is too large for the sweep records (2 GiB)";
+    return GSIS_EUNSUPPORTED;
+  }
   const size_t n_dbl = 4 * N + (size_t)kNRec * nq + 4 * nq + 8 * (size_t)std::max(st.nflux, 1) + 256;
   const size_t bytes = n_dbl * 8 + 64;
   void *mem = nullptr;
