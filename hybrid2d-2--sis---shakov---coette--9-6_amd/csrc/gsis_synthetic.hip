@@ -49,7 +49,7 @@ constexpr int kTW = kTile + 4;               // with two halo rings each side
 constexpr int kTC = kTW * kTW;
 constexpr int kNFace = (kTile + 1) * kTile;  // i-faces (or j-faces) of a tile
 constexpr int kMaxSweepWaves = 16;           // rows <= 1024
-constexpr int kRing = 8;                     // sweep hand-off slots between wavefronts
+constexpr int kRingMax = 10;                 // sweep hand-off slots per wavefront (at most)
 constexpr long long kWaitTicks = 200000000;  // bounded waits: 2 s of the 100 MHz wall clock
 
 enum { kCtlGo = 1, kCtlErr = 3 };
@@ -75,6 +75,7 @@ __device__ __forceinline__ size_t rec_cell(int i, int j, int nw) {  // the cell'
 constexpr int kBufWord3 = 0x00020000;  // gfx9-family descriptor word 3 (raw buffer, stride 0)
 constexpr int kCoherent = 16;          // buffer instruction cache policy: sc1
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const SyntheticState &st) {
   return __builtin_amdgcn_make_buffer_rsrc(st.rec, (short)0, (int)((size_t)st.np * st.nw * kBlk * sizeof(double)),
                                            kBufWord3);
@@ -197,9 +198,10 @@ struct NsShared {
   double mu[kTC];      // amu_of
   double fi[4][kNFace];  // i-face fluxes, face (i0+a, j0+b) at b*(kTile+1)+a
   double fj[4][kNFace];  // j-face fluxes, face (i0+a, j0+b) at b*kTile+a
-  // sweep: lane 63 (forward) or lane 0 (backward) hands its row's values of
-  // step s to the neighbouring wavefront in slot s % kRing
-  double xw[kRing][kMaxSweepWaves][10];
+  // sweep: lane 63 (forward) or lane 0 (backward) of wavefront w hands its
+  // row's values of step s to the neighbouring wavefront in slot s % kRing,
+  // at xw[(w * kRingMax + slot) * 10]
+  double xw[kMaxSweepWaves * kRingMax * 10];
   int prog[kMaxSweepWaves];  // steps a wavefront has handed on
   int cons[kMaxSweepWaves];  // steps whose hand-off a wavefront has taken
   int abort;                 // a hand-off wait timed out
@@ -357,18 +359,17 @@ __device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, 
 
 __device__ __forceinline__ void barrier_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// x of lane l-1 (wave_shr:1) or l+1 (wave_shl:1) of the wavefront; edge in
-// the lane that has no such neighbour (lane 0, or lane 63)
-__device__ __forceinline__ double lane_shift(double x, bool up, double edge) {
-  const unsigned long long b = __double_as_longlong(x), e = __double_as_longlong(edge);
+// x of lane l-1 (wave_shr:1) or l+1 (wave_shl:1) of the wavefront; 0 in the
+// lane that has no such neighbour (lane 0, or lane 63)
+__device__ __forceinline__ double lane_shift(double x, bool up) {
+  const unsigned long long b = __double_as_longlong(x);
   int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
-  const int elo = (int)(unsigned)e, ehi = (int)(unsigned)(e >> 32);
   if (up) {
-    lo = __builtin_amdgcn_update_dpp(elo, lo, 0x138, 0xf, 0xf, false);
-    hi = __builtin_amdgcn_update_dpp(ehi, hi, 0x138, 0xf, 0xf, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x138, 0xf, 0xf, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x138, 0xf, 0xf, true);
   } else {
-    lo = __builtin_amdgcn_update_dpp(elo, lo, 0x130, 0xf, 0xf, false);
-    hi = __builtin_amdgcn_update_dpp(ehi, hi, 0x130, 0xf, 0xf, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x130, 0xf, 0xf, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
   }
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
@@ -383,8 +384,8 @@ struct Rec {
 // cell there (the slot exists for every row of a sweep wavefront): nine
 // 16-byte loads, one scalar block offset and immediate pair offsets.
 template <bool FWD>
-__device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t rr, int blk_bytes, const double2 *ds, int lane,
-                                         Rec &x) {
+__device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t rr, int blk_bytes, __amdgpu_buffer_rsrc_t rd,
+                                         int ds_bytes, int lane, Rec &x) {
   const unsigned v = (unsigned)lane * 16u;
   auto pr = [&](int q) { return ld_pair(rr, v + (unsigned)q * 1024u, blk_bytes); };
   double2 r01, r23;
@@ -392,8 +393,8 @@ __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t rr, int blk_byte
     r01 = pr(kRecRes / 2);
     r23 = pr(kRecRes / 2 + 1);
   } else {  // dU* was stored by this lane
-    r01 = ds[0];
-    r23 = ds[64];
+    r01 = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rd, v, ds_bytes, 0));
+    r23 = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rd, v + 1024u, ds_bytes, 0));
   }
   const double2 u01 = pr(kRecU / 2), u23 = pr(kRecU / 2 + 1), p4 = pr(4), p5 = pr(5), p6 = pr(6), p7 = pr(7),
                 p8 = pr(8);
@@ -410,6 +411,12 @@ __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t rr, int blk_byte
 __device__ __forceinline__ int lds_get(int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ __forceinline__ void lds_put(int *p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// x in a vector register: LDS addresses formed from it keep their constant
+// parts in the instructions' offset fields
+__device__ __forceinline__ int in_vgpr(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
 }
 __device__ __forceinline__ void lgkm_wait() { __builtin_amdgcn_s_waitcnt(0xC07F); }  // lgkmcnt(0)
 __device__ __forceinline__ void mem_order() { asm volatile("" ::: "memory"); }  // no compiler reordering
@@ -430,23 +437,21 @@ __device__ __forceinline__ void wait_lds(int *p, int want, int *abort, int *err)
   }
 }
 
-// What a lane carries from one step to the next: its own cell's U, Cbi,
-// Cbj, Hsi, Hsj and dU.  The lane's next cell is the i-neighbour of this one
-// and the next row's lane reads it as its j-neighbour.
-struct Carry {
-  double u[4], cbi, cbj, hsi, hsj, du[4];
-};
-
 // du_LU_SGS_2D (src/sub_LU_SGS.f90:4-113) and the update of cells 3..n-3.
 // Lane j owns row j (j = 3 + thread index); step p visits plane i + j = p.
 // Each step is straight-line code: every lane issues the same loads and
-// stores (a lane without a cell stores to the dump), so the waits for a
-// record loaded kDepth steps earlier do not wait for the later ones.  The
-// loop runs a multiple of kDepth steps; the steps after the last plane have
-// no cell in any row.
+// stores (a lane without a cell stores out of the buffer's range, which
+// discards the store), so the waits for a record loaded kDepth steps earlier
+// do not wait for the later ones.  The loop runs a multiple of kDepth steps;
+// the steps after the last plane have no cell in any row.
 // kDepth records in flight: each step's record is loaded kDepth steps before
 // it is used (their loads come from the device's coherence point, written by
 // other CUs), which takes (kDepth - 1) * 34 more VGPRs than one record.
+//
+// What a lane carries from one step to the next: its own cell's U, dU, Cbi
+// and Hsi (the cell is the i-neighbour of the lane's next one), and the
+// j-neighbour row's U, dU, Cbj and Hsj, shifted in from the next lane at the
+// end of the step (in the edge lane: from the neighbouring wavefront, below).
 //
 // Hand-offs between wavefronts (LDS ring of kRing slots per wavefront): the
 // edge lane (63 forward, 0 backward) of wavefront w writes its values of step
@@ -454,9 +459,10 @@ struct Carry {
 // cons[w] = s + 1 that wavefront w has taken the slots up to s of its
 // neighbour.  No step waits for an LDS round trip in the steady state:
 //  * the words and the neighbour's next slot are read at the end of a step,
-//    one step before they are needed (first the word, then the slot: the LDS
-//    performs a wavefront's operations in order, so a slot read after a word
-//    that says "written" holds the written values);
+//    one step before they are needed, the slot straight into the edge lane of
+//    the shifted row (first the word, then the slot: the LDS performs a
+//    wavefront's operations in order, so a slot read after a word that says
+//    "written" holds the written values);
 //  * the next step first works on its own i-neighbour, then waits for those
 //    reads (lgkmcnt(0)), publishes prog and cons, and only if the word it read
 //    was too old waits for the neighbour and reads the slot again.  A
@@ -477,74 +483,80 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   // hands its edge row to
   const int from = FWD ? w - 1 : w + 1, to = FWD ? w + 1 : w - 1;
   const bool has_from = from >= 0 && from < nwr, has_to = to >= 0 && to < nwr;
-  int *prog = sh.prog, *cons = sh.cons, *abort = &sh.abort, *err = st.ctl + kCtlErr;
+  const bool edge_in = FWD ? lane == 0 : lane == 63, edge_out = FWD ? lane == 63 : lane == 0;
+  constexpr int kRing = 2 * kDepth;  // a multiple of kDepth: no slot index wraps within a group of steps
+  static_assert(kRing <= kRingMax, "hand-off ring");
+  int *abort = &sh.abort, *err = st.ctl + kCtlErr;
+  const int fromc = has_from ? from : w, toc = has_to ? to : w;  // (in range also where unused)
+  int *pw = &sh.prog[in_vgpr(w)], *pfrom = &sh.prog[in_vgpr(fromc)];  // prog of this wavefront, of from
+  int *cw = &sh.cons[in_vgpr(w)], *cto = &sh.cons[in_vgpr(toc)];      // cons of this wavefront, of to
+  // records, dU* (pairs of components, 256 doubles per block) and U as raw buffers
   const __amdgpu_buffer_rsrc_t rr = rec_rsrc(st);
-  const double2 *dstar = reinterpret_cast<const double2 *>(st.dstar);
-  double2 *dump2 = reinterpret_cast<double2 *>(st.dump) + lane;  // [2][64] pairs
-  double *dump1 = st.dump + lane;                                  // [4][64]
+  const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(
+      st.dstar, (short)0, (int)((size_t)st.np * nwr * 256 * sizeof(double)), kBufWord3);
+  const __amdgpu_buffer_rsrc_t ru =
+      __builtin_amdgcn_make_buffer_rsrc(d.U, (short)0, (int)((size_t)4 * N * sizeof(double)), kBufWord3);
+  constexpr unsigned kOff = 0x80000000u;  // past every buffer's range: the store is discarded
   // the plane of step s (clamped to the last one: the steps past it reload it)
   // as a block index (p - 6) * nw + w
   auto bidx = [&](int s) {
     const int q = min(s, nstep - 1);
     return (FWD ? q : nstep - 1 - q) * nwr + w;
   };
-  auto blk_bytes = [&](int s) { return bidx(s) * (kBlk * (int)sizeof(double)); };
-  auto dsb = [&](int s) { return dstar + (size_t)bidx(s) * 128 + lane; };
   auto pof = [&](int s) { return FWD ? pmin + s : pmax - s; };
   auto act = [&](int s) {
     const int p = pof(s);
     return s < nstep && j <= ny - 3 && p - j >= 3 && p - j <= nx - 3;
   };
-  // the neighbour's slot of step t: U, dU, Cbj, Hsj of its edge row
-  auto read_slot = [&](int t, double e[10]) {
-    const double *xb = sh.xw[t % kRing][from];
-#pragma unroll
-    for (int m = 0; m < 10; m++) e[m] = xb[m];
+  // byte offset of cell (p - j, j) in a component of U, less 8 p
+  const unsigned ucell = (unsigned)(((j + 1) * d.NI + 1 - j) * (int)sizeof(double));
+  // own previous cell (the i-neighbour) and the j-neighbour row, both of the previous step
+  double cu[4], cdu[4], ccbi = 0.0, chsi = 0.0;
+  double nu[4], ndu[4], ncb = 0.0, nhs = 0.0;
+  for (int m = 0; m < 4; m++) { cu[m] = nu[m] = 1.0; cdu[m] = ndu[m] = 0.0; }
+  // the neighbour's slot at xw[k] into the edge lane of the j-neighbour row
+  auto read_slot = [&](int k) {
+    if (edge_in) {
+      const double *xb = &sh.xw[k];
+      for (int m = 0; m < 4; m++) { nu[m] = xb[m]; ndu[m] = xb[4 + m]; }
+      ncb = xb[8];
+      nhs = xb[9];
+    }
   };
-  Carry c;
-  for (int m = 0; m < 4; m++) { c.u[m] = 1.0; c.du[m] = 0.0; }
-  c.cbi = c.cbj = c.hsi = c.hsj = 0.0;
-  double e[10];  // the neighbour's edge row of the previous step (0 without a neighbour)
-#pragma unroll
-  for (int m = 0; m < 10; m++) e[m] = 0.0;
+  auto slot_of = [&](int v, int t) { return (v * kRingMax + t % kRing) * 10; };
   int fprog = 0, fcons = 0;  // prog[from] and cons[to] as read one step earlier
   Rec r[kDepth];
 #pragma unroll
-  for (int q = 0; q < kDepth; q++) load_rec<FWD>(rr, blk_bytes(q), dsb(q), lane, r[q]);
-  auto step = [&](int s, Rec &x) {
+  for (int q = 0; q < kDepth; q++) load_rec<FWD>(rr, bidx(q) * kBlk * 8, rd, bidx(q) * 2048, lane, r[q]);
+  // slots of the group of steps s0 .. s0+kDepth-1: xin + 10 q (from's), xout + 10 q (this wavefront's)
+  int xin = 0, xout = 0;
+  auto step = [&](int s, int q, Rec &x) {
     const int p = pof(s), i = p - j;
     const bool on = act(s);
     // off-diagonal term from the i-neighbour (this lane's previous cell)
     double dfi[4], dfj[4];
-    dfn(c.u, c.du, x.ni[0], x.ni[1], g, dfi);
+    dfn(cu, cdu, x.ni[0], x.ni[1], g, dfi);
     if (s > 0) {
       lgkm_wait();  // the previous step's slot write and reads are complete
-      if (has_to && lane == 0) lds_put(prog + w, s);
+      if (has_to && lane == 0) lds_put(pw, s);
       if (has_from) {
         if (__builtin_amdgcn_readfirstlane(fprog) < s) {  // read too early: wait for slot s-1, read it again
-          wait_lds(prog + from, s, abort, err);
+          wait_lds(pfrom, s, abort, err);
           mem_order();
-          read_slot(s - 1, e);
+          read_slot(slot_of(fromc, s - 1));
           lgkm_wait();
         }
-        if (lane == 0) lds_put(cons + w, s);
+        if (lane == 0) lds_put(cw, s);
       }
     }
-    // the j-neighbour: the lane below (forward) or above (backward), the
-    // neighbouring wavefront's edge row in the edge lane
-    double nu[4], ndu[4];
-    for (int m = 0; m < 4; m++) {
-      nu[m] = lane_shift(c.u[m], FWD, e[m]);
-      ndu[m] = lane_shift(c.du[m], FWD, e[4 + m]);
-    }
-    const double ncb = lane_shift(c.cbj, FWD, e[8]), nhs = lane_shift(c.hsj, FWD, e[9]);
+    // off-diagonal term from the j-neighbour
     dfn(nu, ndu, x.nj[0], x.nj[1], g, dfj);
     const bool usei = FWD ? i != 3 : i != nx - 3, usej = FWD ? j != 3 : j != ny - 3;
     double du[4];
     for (int m = 0; m < 4; m++) {
       // forward:  dui = 0.5 (DF si(i,j) + Lci(i-1,j) dU) + Lvi(i-1,j) dU      (:44-49)
       // backward: dui = -0.5 (DF si(i+1,j) - Lci(i+1,j) dU) + Lvi(i+1,j) dU   (:82-86)
-      const double ti = FWD ? fma(x.hsi, dfi[m], c.cbi * c.du[m]) : fma(-c.hsi, dfi[m], c.cbi * c.du[m]);
+      const double ti = FWD ? fma(x.hsi, dfi[m], ccbi * cdu[m]) : fma(-chsi, dfi[m], ccbi * cdu[m]);
       const double tj = FWD ? fma(x.hsj, dfj[m], ncb * ndu[m]) : fma(-nhs, dfj[m], ncb * ndu[m]);
       double a = FWD ? x.res[m] : 0.0;
       a = usei ? a + ti : a;
@@ -554,53 +566,60 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     // plain stores: dU* is read back by this lane, U is written back to the
     // device's coherence point by the release fence after the sweep
     if (FWD) {
-      double2 *o = on ? reinterpret_cast<double2 *>(st.dstar) + (size_t)bidx(s) * 128 + lane : dump2;
-      o[0] = double2{du[0], du[1]};
-      o[64] = double2{du[2], du[3]};
+      const unsigned v = on ? (unsigned)lane * 16u : kOff;
+      const int so = bidx(s) * 2048;
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, double2{du[0], du[1]}), rd, v, so, 0);
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, double2{du[2], du[3]}), rd, v + 1024u, so,
+                                             0);
     } else {
-      for (int m = 0; m < 4; m++) {
-        double *o = on ? d.U + m * N + cidx(d, i, j) : dump1 + m * 64;
-        *o = x.u[m] + du[m];
-      }
+      const unsigned v = on ? ucell + (unsigned)p * 8u : kOff;
+      for (int m = 0; m < 4; m++)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x.u[m] + du[m]), ru, v,
+                                              m * N * (int)sizeof(double), 0);
     }
+    // carry: this cell, and the j-neighbour row of the next step
     for (int m = 0; m < 4; m++) {
-      c.u[m] = x.u[m];
-      c.du[m] = on ? du[m] : 0.0;
+      cu[m] = x.u[m];
+      cdu[m] = on ? du[m] : 0.0;
+      nu[m] = lane_shift(cu[m], FWD);
+      ndu[m] = lane_shift(cdu[m], FWD);
     }
-    c.cbi = x.cbi;
-    c.cbj = x.cbj;
-    c.hsi = x.hsi;
-    c.hsj = x.hsj;
+    ccbi = x.cbi;
+    chsi = x.hsi;
+    ncb = lane_shift(x.cbj, FWD);
+    nhs = lane_shift(x.hsj, FWD);
     if (has_to) {  // hand the edge row on, once slot s % kRing has been taken
       if (__builtin_amdgcn_readfirstlane(fcons) < s - kRing + 1) {
-        wait_lds(cons + to, s - kRing + 1, abort, err);
+        wait_lds(cto, s - kRing + 1, abort, err);
         mem_order();
       }
-      if (FWD ? lane == 63 : lane == 0) {
-        double *xb = sh.xw[s % kRing][w];
-        for (int m = 0; m < 4; m++) { xb[m] = c.u[m]; xb[4 + m] = c.du[m]; }
-        xb[8] = c.cbj;
-        xb[9] = c.hsj;
+      if (edge_out) {
+        double *xb = &sh.xw[xout + 10 * q];
+        for (int m = 0; m < 4; m++) { xb[m] = cu[m]; xb[4 + m] = cdu[m]; }
+        xb[8] = x.cbj;
+        xb[9] = x.hsj;
       }
       mem_order();
-      fcons = lds_get(cons + to);
+      fcons = lds_get(cto);
     }
     if (has_from) {  // for the next step: the word, then the slot of this step
       mem_order();
-      fprog = lds_get(prog + from);
+      fprog = lds_get(pfrom);
       mem_order();
-      read_slot(s, e);
+      read_slot(xin + 10 * q);
     }
-    load_rec<FWD>(rr, blk_bytes(s + kDepth), dsb(s + kDepth), lane, x);
+    load_rec<FWD>(rr, bidx(s + kDepth) * kBlk * 8, rd, bidx(s + kDepth) * 2048, lane, x);
   };
   for (int s = 0; s < nstepd; s += kDepth) {
+    xin = in_vgpr(slot_of(fromc, s));
+    xout = in_vgpr(slot_of(w, s));
 #pragma unroll
-    for (int q = 0; q < kDepth; q++) step(s + q, r[q]);
+    for (int q = 0; q < kDepth; q++) step(s + q, q, r[q]);
   }
 }
 
 // The wavefronts of a pass are coupled only through their hand-offs: a
-// wavefront may run up to kRing steps ahead of the one that takes its edge row.
+// wavefront may run up to 2 kDepth steps ahead of the one that takes its edge row.
 template <int kDepth>
 __device__ void sweep(const Dev &d, const SyntheticState &st, NsShared &sh) {
   const int tid = threadIdx.x;
@@ -848,8 +867,9 @@ int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st) {
   }
   st.nw = (st.rs + 63) / 64;
   const size_t nq = (size_t)st.np * st.nw * 64;  // cells of the blocked sweep layout
-  if ((size_t)kNRec * nq * sizeof(double) >= ((size_t)1 << 31)) {  // 32-bit buffer offsets of the records
-    g_syn_err = "synthetic solve: the mesh is too large for the sweep records (2 GiB)";
+  // 32-bit buffer offsets of the records and of U
+  if ((size_t)kNRec * nq * sizeof(double) >= ((size_t)1 << 31) || (size_t)4 * N * sizeof(double) >= ((size_t)1 << 31)) {
+    g_syn_err = "synthetic solve: the mesh is too large for the sweep's buffers (2 GiB)";
     return GSIS_EUNSUPPORTED;
   }
   const size_t n_dbl = 4 * N + (size_t)kNRec * nq + 4 * nq + 8 * (size_t)std::max(st.nflux, 1) + 256;
