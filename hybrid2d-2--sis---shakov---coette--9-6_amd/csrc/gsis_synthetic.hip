@@ -193,15 +193,16 @@ __device__ __forceinline__ void dfn(const double u[4], const double du[4], doubl
 }
 
 struct NsShared {
+  // sweep: lane 63 (forward) or lane 0 (backward) of wavefront w hands its
+  // row's values of step s to the neighbouring wavefront in slot s % kRing,
+  // at xw[(w * kRingMax + slot) * 10] (first in the struct: slot addresses
+  // then fit the LDS instructions' offset fields)
+  double xw[kMaxSweepWaves * kRingMax * 10];
   double U[4][kTC];    // tile + halo
   double pr[5][kTC];   // d, u, v, T, p (comput_duvtpckw)
   double mu[kTC];      // amu_of
   double fi[4][kNFace];  // i-face fluxes, face (i0+a, j0+b) at b*(kTile+1)+a
   double fj[4][kNFace];  // j-face fluxes, face (i0+a, j0+b) at b*kTile+a
-  // sweep: lane 63 (forward) or lane 0 (backward) of wavefront w hands its
-  // row's values of step s to the neighbouring wavefront in slot s % kRing,
-  // at xw[(w * kRingMax + slot) * 10]
-  double xw[kMaxSweepWaves * kRingMax * 10];
   int prog[kMaxSweepWaves];  // steps a wavefront has handed on
   int cons[kMaxSweepWaves];  // steps whose hand-off a wavefront has taken
   int abort;                 // a hand-off wait timed out
@@ -514,10 +515,9 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   double cu[4], cdu[4], ccbi = 0.0, chsi = 0.0;
   double nu[4], ndu[4], ncb = 0.0, nhs = 0.0;
   for (int m = 0; m < 4; m++) { cu[m] = nu[m] = 1.0; cdu[m] = ndu[m] = 0.0; }
-  // the neighbour's slot at xw[k] into the edge lane of the j-neighbour row
-  auto read_slot = [&](int k) {
+  // the neighbour's slot at xb into the edge lane of the j-neighbour row
+  auto read_slot = [&](const double *xb) {
     if (edge_in) {
-      const double *xb = &sh.xw[k];
       for (int m = 0; m < 4; m++) { nu[m] = xb[m]; ndu[m] = xb[4 + m]; }
       ncb = xb[8];
       nhs = xb[9];
@@ -528,8 +528,13 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   Rec r[kDepth];
 #pragma unroll
   for (int q = 0; q < kDepth; q++) load_rec<FWD>(rr, bidx(q) * kBlk * 8, rd, bidx(q) * 2048, lane, r[q]);
+  // block of the loads issued in step s (those of step s + kDepth), moving by
+  // one plane per step and held at the last plane; block of step s
+  const int dbl = FWD ? nwr : -nwr, blim = FWD ? (nstep - 1) * nwr + w : w;
+  int bl = bidx(kDepth), bs = w;
   // slots of the group of steps s0 .. s0+kDepth-1: xin + 10 q (from's), xout + 10 q (this wavefront's)
-  int xin = 0, xout = 0;
+  const double *xin = sh.xw;
+  double *xout = sh.xw;
   auto step = [&](int s, int q, Rec &x) {
     const int p = pof(s), i = p - j;
     const bool on = act(s);
@@ -543,7 +548,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
         if (__builtin_amdgcn_readfirstlane(fprog) < s) {  // read too early: wait for slot s-1, read it again
           wait_lds(pfrom, s, abort, err);
           mem_order();
-          read_slot(slot_of(fromc, s - 1));
+          read_slot(&sh.xw[slot_of(fromc, s - 1)]);
           lgkm_wait();
         }
         if (lane == 0) lds_put(cw, s);
@@ -567,7 +572,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     // device's coherence point by the release fence after the sweep
     if (FWD) {
       const unsigned v = on ? (unsigned)lane * 16u : kOff;
-      const int so = bidx(s) * 2048;
+      const int so = bs * 2048;  // bidx(s) where a lane has a cell
       __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, double2{du[0], du[1]}), rd, v, so, 0);
       __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, double2{du[2], du[3]}), rd, v + 1024u, so,
                                              0);
@@ -594,7 +599,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
         mem_order();
       }
       if (edge_out) {
-        double *xb = &sh.xw[xout + 10 * q];
+        double *xb = xout + 10 * q;
         for (int m = 0; m < 4; m++) { xb[m] = cu[m]; xb[4 + m] = cdu[m]; }
         xb[8] = x.cbj;
         xb[9] = x.hsj;
@@ -608,11 +613,13 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       mem_order();
       read_slot(xin + 10 * q);
     }
-    load_rec<FWD>(rr, bidx(s + kDepth) * kBlk * 8, rd, bidx(s + kDepth) * 2048, lane, x);
+    load_rec<FWD>(rr, bl * (kBlk * 8), rd, bl * 2048, lane, x);
+    bl = FWD ? min(bl + dbl, blim) : max(bl + dbl, blim);
+    bs += nwr;
   };
   for (int s = 0; s < nstepd; s += kDepth) {
-    xin = in_vgpr(slot_of(fromc, s));
-    xout = in_vgpr(slot_of(w, s));
+    xin = &sh.xw[in_vgpr(slot_of(fromc, s))];
+    xout = &sh.xw[in_vgpr(slot_of(w, s))];
 #pragma unroll
     for (int q = 0; q < kDepth; q++) step(s + q, q, r[q]);
   }
@@ -663,42 +670,69 @@ __device__ bool wait_until(const int *p, int want, bool ge, int *err) {
 // compute the fluxes; see the file comment.  Flux workgroup w reports an
 // iteration in its own slot part[8w..]: its max |Res_4| and T<0.1 flag, then
 // the iteration number once those stores are complete.
+// The flux workgroups' part of the inner loop.  A separate (not inlined)
+// function: the sweep's registers are then allocated without the flux code's
+// live values, which otherwise push the sweep's scalar registers into
+// spills.
+// A copy of x with every 32-bit word read once and made wavefront-uniform
+// (scalar registers): x is a function argument passed through the stack,
+// whose fields would otherwise be reloaded per use into vector registers.
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T &x) {
+  static_assert(sizeof(T) % 4 == 0, "word copy");
+  T y;
+  const int *src = reinterpret_cast<const int *>(&x);
+  int *dst = reinterpret_cast<int *>(&y);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); k++) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+  return y;
+}
+
+__device__ __attribute__((noinline)) void ns_flux_role(const Dev d_arg, const SyntheticState st_arg, NsShared &sh) {
+  const Dev d = uniform_copy(d_arg);
+  const SyntheticState st = uniform_copy(st_arg);
+  int *ctl = st.ctl;
+  const int tid = threadIdx.x;
+  const int wg = blockIdx.x - 1, ntiles = st.ntx * st.nty;
+  int *slot = reinterpret_cast<int *>(st.part + 8 * wg);
+  for (int k = st.k0;; k++) {
+    if (tid == 0) {
+      int go = -1;
+      if (wait_until<60>(ctl + kCtlGo, k, false, ctl + kCtlErr)) go = ld_agent(ctl + kCtlGo);
+      sh.word = go == k ? 1 : 0;
+    }
+    __syncthreads();
+    const int run = sh.word;
+    __syncthreads();
+    if (!run) return;
+    double rmax = 0.0;
+    int bad = 0;
+    for (int tile = wg; tile < ntiles; tile += st.nflux) flux_tile(d, st, sh, tile, rmax, bad);
+    // the workgroup's max |Res_4| and T<0.1 flag
+    for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_down(rmax, off, 64));
+    bad = __syncthreads_or(bad);
+    if ((tid & 63) == 0) sh.red[tid >> 6] = rmax;
+    __syncthreads();
+    if (tid == 0) {
+      double m = sh.red[0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = fmax(m, sh.red[q]);
+      st_agent(st.part + 8 * wg + 1, m);
+      st_agent(slot + 1, bad);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores are complete
+    __syncthreads();
+    if (tid == 0) st_agent(slot, k);
+  }
+}
+
 template <int B>
 __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
   __shared__ NsShared sh;
   int *ctl = st.ctl;
   const int tid = threadIdx.x;
   if (blockIdx.x > 0) {
-    const int wg = blockIdx.x - 1, ntiles = st.ntx * st.nty;
-    int *slot = reinterpret_cast<int *>(st.part + 8 * wg);
-    for (int k = st.k0;; k++) {
-      if (tid == 0) {
-        int go = -1;
-        if (wait_until<60>(ctl + kCtlGo, k, false, ctl + kCtlErr)) go = ld_agent(ctl + kCtlGo);
-        sh.word = go == k ? 1 : 0;
-      }
-      __syncthreads();
-      const int run = sh.word;
-      __syncthreads();
-      if (!run) return;
-      double rmax = 0.0;
-      int bad = 0;
-      for (int tile = wg; tile < ntiles; tile += st.nflux) flux_tile(d, st, sh, tile, rmax, bad);
-      // the workgroup's max |Res_4| and T<0.1 flag
-      for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_down(rmax, off, 64));
-      bad = __syncthreads_or(bad);
-      if ((tid & 63) == 0) sh.red[tid >> 6] = rmax;
-      __syncthreads();
-      if (tid == 0) {
-        double m = sh.red[0];
-        for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = fmax(m, sh.red[q]);
-        st_agent(st.part + 8 * wg + 1, m);
-        st_agent(slot + 1, bad);
-      }
-      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores are complete
-      __syncthreads();
-      if (tid == 0) st_agent(slot, k);
-    }
+    ns_flux_role(d, st, sh);
+    return;
   }
   // workgroup 0: the LU-SGS sweeps and the loop control
   int it = st.k0 - 1, bad = 0, stopped = 0;
