@@ -76,6 +76,12 @@ constexpr int kBufWord3 = 0x00020000;  // gfx9-family descriptor word 3 (raw buf
 constexpr int kCoherent = 16;          // buffer instruction cache policy: sc1
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Global-memory views of the mesh arrays: loads through them are global
+// loads also where the compiler cannot infer the address space of a Dev
+// field (the flux workgroups' function, the batched kernel's job table)
+typedef const __attribute__((address_space(1))) double gdbl;
+__device__ __forceinline__ gdbl *gp(const double *p) { return (gdbl *)p; }
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const SyntheticState &st) {
   return __builtin_amdgcn_make_buffer_rsrc(st.rec, (short)0, (int)((size_t)st.np * st.nw * kBlk * sizeof(double)),
                                            kBufWord3);
@@ -220,8 +226,8 @@ struct TileCells {
   __device__ double v(int i, int j) const { return sh.pr[2][l(i, j)]; }
   __device__ double T(int i, int j) const { return sh.pr[3][l(i, j)]; }
   __device__ double mu(int i, int j) const { return sh.mu[l(i, j)]; }
-  __device__ double x1(int i, int j) const { return d.x1[cidx(d, i, j)]; }
-  __device__ double y1(int i, int j) const { return d.y1[cidx(d, i, j)]; }
+  __device__ double x1(int i, int j) const { return gp(d.x1)[cidx(d, i, j)]; }
+  __device__ double y1(int i, int j) const { return gp(d.y1)[cidx(d, i, j)]; }
 };
 
 // Residual_FVM face flux (src/sub_Residual.f90:68-202, :210-338) of face
@@ -239,8 +245,8 @@ __device__ void face_flux(const Dev &d, const TileCells &tc, int dir, int i, int
   }
   const int g = gidx(d, i, j);
   double s, n1, n2;
-  if (dir == 0) { s = d.si[g]; n1 = d.ni1[g]; n2 = d.ni2[g]; }
-  else { s = d.sj[g]; n1 = d.nj1[g]; n2 = d.nj2[g]; }
+  if (dir == 0) { s = gp(d.si)[g]; n1 = gp(d.ni1)[g]; n2 = gp(d.ni2)[g]; }
+  else { s = gp(d.sj)[g]; n1 = gp(d.nj1)[g]; n2 = gp(d.nj2)[g]; }
   const double ql[4] = {ul[0], ul[1] * n1 + ul[2] * n2, -ul[1] * n2 + ul[2] * n1, ul[3]};
   const double qr[4] = {ur[0], ur[1] * n1 + ur[2] * n2, -ur[1] * n2 + ur[2] * n1, ur[3]};
   double f0[4];
@@ -253,8 +259,8 @@ __device__ void face_flux(const Dev &d, const TileCells &tc, int dir, int i, int
   double o[8];
   ns_face_of(tc, d, dir, i, j, o);
   const int N = d.ncell, c = cidx(d, i, j);
-  const double *tg = dir == 0 ? d.taoi : d.taoj;
-  const double *qg = dir == 0 ? d.qi : d.qj;
+  gdbl *tg = gp(dir == 0 ? d.taoi : d.taoj);
+  gdbl *qg = gp(dir == 0 ? d.qi : d.qj);
   const double t11 = o[0] + tg[c], t22 = o[1] + tg[N + c], t12 = o[2] + tg[2 * N + c];
   const double qx = o[4] + qg[c], qy = o[5] + qg[N + c];
   const double u1 = o[6], v1 = o[7];
@@ -284,7 +290,7 @@ __device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, 
     const int c = cidx(d, i, j);
     double u[4];
     for (int m = 0; m < 4; m++) {
-      u[m] = ld_agent(d.U + m * N + c);
+      u[m] = __hip_atomic_load(gp(d.U) + m * N + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       sh.U[m][t] = u[m];
     }
     const double dd = u[0], uu = u[1] / dd, vv = u[2] / dd;
@@ -323,11 +329,13 @@ __device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, 
     rmax = fmax(rmax, fabs(res[3]));
     // comput_Lvc and comput_dt (src/sub_NS_singlegrid.f90:160-221)
     const int g0 = gidx(d, i, j), gi = gidx(d, i + 1, j), gj = gidx(d, i, j + 1);
-    const double s0 = d.vol[g0];
-    const double si = 0.5 * (d.si[g0] + d.si[gi]);
-    const double ni1 = 0.5 * (d.ni1[g0] + d.ni1[gi]), ni2 = 0.5 * (d.ni2[g0] + d.ni2[gi]);
-    const double sj = 0.5 * (d.sj[g0] + d.sj[gj]);
-    const double nj1 = 0.5 * (d.nj1[g0] + d.nj1[gj]), nj2 = 0.5 * (d.nj2[g0] + d.nj2[gj]);
+    gdbl *gvol = gp(d.vol), *gsi = gp(d.si), *gsj = gp(d.sj);
+    gdbl *gni1 = gp(d.ni1), *gni2 = gp(d.ni2), *gnj1 = gp(d.nj1), *gnj2 = gp(d.nj2);
+    const double s0 = gvol[g0];
+    const double si = 0.5 * (gsi[g0] + gsi[gi]);
+    const double ni1 = 0.5 * (gni1[g0] + gni1[gi]), ni2 = 0.5 * (gni2[g0] + gni2[gi]);
+    const double sj = 0.5 * (gsj[g0] + gsj[gj]);
+    const double nj1 = 0.5 * (gnj1[g0] + gnj1[gj]), nj2 = 0.5 * (gnj2[g0] + gnj2[gj]);
     const int lc = tc.l(i, j);
     const double uu = sh.pr[1][lc], vv = sh.pr[2][lc];
     const double uni = uu * ni1 + vv * ni2, unj = uu * nj1 + vv * nj2;
@@ -338,11 +346,11 @@ __device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, 
     double dt;
     if (!st.local_dt) dt = d.dt;
     else {
-      dt = st.cfl * d.vol[g0] / (lci + lcj + 1.0 * (lvi + lvj));
+      dt = st.cfl * gvol[g0] / (lci + lcj + 1.0 * (lvi + lvj));
       if (dt > st.dtmax) dt = st.dtmax;
       if (dt < st.dtmin) dt = st.dtmin;
     }
-    double alfa = d.vol[g0] / dt + 1.0 * (lci + lcj);
+    double alfa = gvol[g0] / dt + 1.0 * (lci + lcj);
     if (visc) alfa = alfa + 2.0 * (lvi + lvj);
     const unsigned o = (unsigned)(rec_cell(i, j, nw) * sizeof(double));
     st_pair(rr, o + (kRecRes / 2) * 1024, res[0], res[1]);
@@ -350,10 +358,10 @@ __device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, 
     st_pair(rr, o + (kRecU / 2) * 1024, sh.U[0][lc], sh.U[1][lc]);
     st_pair(rr, o + (kRecU / 2 + 1) * 1024, sh.U[2][lc], sh.U[3][lc]);
     st_pair(rr, o + (kRecRa / 2) * 1024, 1.0 / alfa, 0.5 * lci + lvi);                // ra, cbi
-    st_pair(rr, o + (kRecCbj / 2) * 1024, 0.5 * lcj + lvj, 0.5 * d.si[g0]);           // cbj, hsi
-    st_pair(rr, o + (kRecHsj / 2) * 1024, 0.5 * d.sj[g0], d.ni1[g0]);                 // hsj, ni1
-    st_pair(rr, o + (kRecNi / 2 + 1) * 1024, d.ni2[g0], d.nj1[g0]);                   // ni2, nj1
-    st_pair(rr, o + (kRecNj / 2 + 1) * 1024, d.nj2[g0], 0.0);                         // nj2
+    st_pair(rr, o + (kRecCbj / 2) * 1024, 0.5 * lcj + lvj, 0.5 * gsi[g0]);  // cbj, hsi
+    st_pair(rr, o + (kRecHsj / 2) * 1024, 0.5 * gsj[g0], gni1[g0]);         // hsj, ni1
+    st_pair(rr, o + (kRecNi / 2 + 1) * 1024, gni2[g0], gnj1[g0]);           // ni2, nj1
+    st_pair(rr, o + (kRecNj / 2 + 1) * 1024, gnj2[g0], 0.0);                // nj2
   }
   __syncthreads();  // the next tile reuses the LDS
 }
@@ -445,9 +453,11 @@ __device__ __forceinline__ void wait_lds(int *p, int want, int *abort, int *err)
 // discards the store), so the waits for a record loaded kDepth steps earlier
 // do not wait for the later ones.  The loop runs a multiple of kDepth steps;
 // the steps after the last plane have no cell in any row.
-// kDepth records in flight: each step's record is loaded kDepth steps before
-// it is used (their loads come from the device's coherence point, written by
-// other CUs), which takes (kDepth - 1) * 34 more VGPRs than one record.
+// kDepth records in registers: each step's record is loaded at the start of
+// the step kDepth - 1 steps before it is used (the loads come from the
+// device's coherence point, written by other CUs, and go out among the
+// step's arithmetic), which takes (kDepth - 1) * 34 more VGPRs than one
+// record.
 //
 // What a lane carries from one step to the next: its own cell's U, dU, Cbi
 // and Hsi (the cell is the i-neighbour of the lane's next one), and the
@@ -527,15 +537,19 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   int fprog = 0, fcons = 0;  // prog[from] and cons[to] as read one step earlier
   Rec r[kDepth];
 #pragma unroll
-  for (int q = 0; q < kDepth; q++) load_rec<FWD>(rr, bidx(q) * kBlk * 8, rd, bidx(q) * 2048, lane, r[q]);
-  // block of the loads issued in step s (those of step s + kDepth), moving by
-  // one plane per step and held at the last plane; block of step s
+  for (int q = 0; q < kDepth - 1; q++) load_rec<FWD>(rr, bidx(q) * kBlk * 8, rd, bidx(q) * 2048, lane, r[q]);
+  // block of the loads issued in step s (those of step s + kDepth - 1), moving
+  // by one plane per step and held at the last plane; block of step s
   const int dbl = FWD ? nwr : -nwr, blim = FWD ? (nstep - 1) * nwr + w : w;
-  int bl = bidx(kDepth), bs = w;
+  int bl = bidx(kDepth - 1), bs = w;
   // slots of the group of steps s0 .. s0+kDepth-1: xin + 10 q (from's), xout + 10 q (this wavefront's)
   const double *xin = sh.xw;
   double *xout = sh.xw;
   auto step = [&](int s, int q, Rec &x) {
+    // the record of step s + kDepth - 1, into the slot step s - 1 has used:
+    // issued first, so that the loads go out among this step's arithmetic
+    load_rec<FWD>(rr, bl * (kBlk * 8), rd, bl * 2048, lane, r[(q + kDepth - 1) % kDepth]);
+    bl = FWD ? min(bl + dbl, blim) : max(bl + dbl, blim);
     const int p = pof(s), i = p - j;
     const bool on = act(s);
     // off-diagonal term from the i-neighbour (this lane's previous cell)
@@ -613,8 +627,6 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       mem_order();
       read_slot(xin + 10 * q);
     }
-    load_rec<FWD>(rr, bl * (kBlk * 8), rd, bl * 2048, lane, x);
-    bl = FWD ? min(bl + dbl, blim) : max(bl + dbl, blim);
     bs += nwr;
   };
   for (int s = 0; s < nstepd; s += kDepth) {
@@ -762,7 +774,7 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
       res_last = res;
       __syncthreads();
       if (bad) break;
-      sweep<B <= 256 ? 5 : 3>(d, st, sh);  // deeper prefetch where registers allow
+      sweep<B <= 256 ? 4 : 3>(d, st, sh);  // deeper prefetch where registers allow
       __syncthreads();
       bool more = res > st.tol && k < st.maxit;
       if (more && k >= st.kstop) {  // the launch ends here; a later one continues at k + 1
@@ -814,7 +826,7 @@ __global__ void __launch_bounds__(256) k_ns_batch(const NsJob *__restrict__ jobs
       bad = b;
       __syncthreads();
       if (bad) break;
-      sweep<5>(d, st, sh);
+      sweep<4>(d, st, sh);
       __syncthreads();
       if (!(res > st.tol && k < st.maxit)) break;
     }
