@@ -20,8 +20,8 @@
 //  * workgroup 0 runs the two sweeps.  Lane j of its wavefronts owns row j,
 //    so one sweep step is one anti-diagonal plane: the forward recursion takes
 //    dU(i-1,j) from the lane's own previous step and dU(i,j-1) from the lane
-//    below (a cross-lane shift, through LDS between wavefronts), with one
-//    workgroup barrier per plane.  The backward pass writes U + dU of each
+//    below (a cross-lane shift; between wavefronts a hand-off through an LDS
+//    ring, no barrier per plane).  The backward pass writes U + dU of each
 //    cell as soon as it is final; the sweeps read the pre-update U from the
 //    records, so the update is the reference's "after both sweeps".
 // Any order of the cells that respects the sweep dependences gives the same
