@@ -678,14 +678,6 @@ __device__ bool wait_until(const int *p, int want, bool ge, int *err) {
   return true;
 }
 
-// The inner loop (src/Hybrid_Sub.f90:99-104): workgroup 0 sweeps, the others
-// compute the fluxes; see the file comment.  Flux workgroup w reports an
-// iteration in its own slot part[8w..]: its max |Res_4| and T<0.1 flag, then
-// the iteration number once those stores are complete.
-// The flux workgroups' part of the inner loop.  A separate (not inlined)
-// function: the sweep's registers are then allocated without the flux code's
-// live values, which otherwise push the sweep's scalar registers into
-// spills.
 // A copy of x with every 32-bit word read once and made wavefront-uniform
 // (scalar registers): x is a function argument passed through the stack,
 // whose fields would otherwise be reloaded per use into vector registers.
@@ -700,6 +692,15 @@ __device__ __forceinline__ T uniform_copy(const T &x) {
   return y;
 }
 
+// The flux workgroups' part of the inner loop.  A separate (not inlined)
+// function: the sweep's registers are then allocated without the flux code's
+// live values, which otherwise push the sweep's scalar registers into
+// spills.  One copy per launch size B, so that each gets the register budget
+// of its kernel's workgroup size (one shared copy is held to 1024 threads).
+// Flux workgroup w reports an iteration in its own slot part[8w..]: its max
+// |Res_4| and T<0.1 flag, then the iteration number once those stores are
+// complete.
+template <int B>
 __device__ __attribute__((noinline)) void ns_flux_role(const Dev d_arg, const SyntheticState st_arg, NsShared &sh) {
   const Dev d = uniform_copy(d_arg);
   const SyntheticState st = uniform_copy(st_arg);
@@ -737,13 +738,15 @@ __device__ __attribute__((noinline)) void ns_flux_role(const Dev d_arg, const Sy
   }
 }
 
+// The inner loop (src/Hybrid_Sub.f90:99-104): workgroup 0 sweeps, the others
+// compute the fluxes; see the file comment.
 template <int B>
 __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
   __shared__ NsShared sh;
   int *ctl = st.ctl;
   const int tid = threadIdx.x;
   if (blockIdx.x > 0) {
-    ns_flux_role(d, st, sh);
+    ns_flux_role<B>(d, st, sh);
     return;
   }
   // workgroup 0: the LU-SGS sweeps and the loop control
