@@ -199,7 +199,7 @@ def bench_batch(pkg, solver, args):
                                   "NND fluxes, kinetic step (mode M0)") + ", one hipGraph replay per step",
                    "instances": len(cases), "cell_velocities": units, "mode": args.mode},
         "autotuner": {"columns_per_workgroup": nw, "order": "largest first" if order else "as drawn",
-                      "ms_per_step": tune_ms},
+                      "ms_per_step": tune_ms, "synthetic_solves_in_graph": b.ns_graph()},
         "inner_iterations_last_step": {"max": max(its), "mean": sum(its) / len(its)} if its else None,
         "roofline": {"bound": "hbm", "kernel": "batched step", "achieved": alg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / HBM_PEAK_GBS, "traffic": None,

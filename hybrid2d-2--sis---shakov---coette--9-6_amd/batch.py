@@ -43,8 +43,9 @@ def instances(count: int, seed: int = 20261202, side=(64, 256), nv=(24, 64), kn=
 
 class Batch:
     """A gsis_batch over GsisSolver handles (one precision and flux scheme;
-    M1 instances run their synthetic solve after the captured part of each
-    step).  The solvers stay usable on their own between batch steps."""
+    the synthetic solves of M1 instances run inside the step's graph, one
+    workgroup each, or after it, one cooperative launch each).  The solvers
+    stay usable on their own between batch steps."""
 
     def __init__(self, solvers):
         self.lib = capi.load_library()
@@ -66,6 +67,16 @@ class Batch:
         capi.check(self.lib, self.lib.gsis_batch_tune(self.b, int(steps_per_trial), ctypes.byref(nw),
                                                       ctypes.byref(order), ctypes.byref(ms)))
         return nw.value, order.value, ms.value
+
+    def set_ns_graph(self, mode: int):
+        """Synthetic solves inside the graph: -1 automatic, 0 after the graph, 1 inside (gsis_batch_set_ns_graph)."""
+        capi.check(self.lib, self.lib.gsis_batch_set_ns_graph(self.b, int(mode)))
+
+    def ns_graph(self) -> int:
+        """The number of instances whose synthetic solve runs inside the graph."""
+        n = ctypes.c_int32()
+        capi.check(self.lib, self.lib.gsis_batch_ns_graph(self.b, ctypes.byref(n)))
+        return n.value
 
     def close(self):
         if getattr(self, "b", None) is not None and self.b.value:

@@ -77,6 +77,8 @@ EXPORTED = (
     "gsis_batch_create",
     "gsis_batch_step",
     "gsis_batch_tune",
+    "gsis_batch_set_ns_graph",
+    "gsis_batch_ns_graph",
     "gsis_batch_destroy",
 )
 
@@ -209,6 +211,8 @@ def load_library(path: os.PathLike | str | None = None) -> ctypes.CDLL:
     lib.gsis_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
     lib.gsis_batch_step.argtypes = [ctypes.c_void_p, ctypes.c_int32]
     lib.gsis_batch_tune.argtypes = [ctypes.c_void_p, ctypes.c_int32, i32p, i32p, c_dp]
+    lib.gsis_batch_set_ns_graph.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    lib.gsis_batch_ns_graph.argtypes = [ctypes.c_void_p, i32p]
     lib.gsis_batch_destroy.argtypes = [ctypes.c_void_p]
     for name in EXPORTED:
         if name != "gsis_last_error":

@@ -154,6 +154,16 @@ module gsis_capi
       integer(c_int32_t), intent(out) :: nw, order
       real(c_double), intent(out) :: ms_per_step
     end function
+    integer(c_int) function gsis_batch_set_ns_graph(b, mode) bind(C, name="gsis_batch_set_ns_graph")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: mode
+    end function
+    integer(c_int) function gsis_batch_ns_graph(b, count) bind(C, name="gsis_batch_ns_graph")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: b
+      integer(c_int32_t), intent(out) :: count
+    end function
     integer(c_int) function gsis_batch_destroy(b) bind(C, name="gsis_batch_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: b

@@ -314,10 +314,13 @@ int gsis_fill_frd_uniform(gsis_handle *h, const double *g, const double *hv);
  * launch of the column-wavefront kernel over every instance's strips, and
  * the step of all instances up to the synthetic solve (S0, kinetic BCs, that
  * launch, moments, DETFPwall, Convert_NS, HoT) is captured once as a hipGraph
- * and replayed per step.  Instances in mode M1 then run their synthetic
- * solve (with the closure when one is set), one after another; it is not
- * captured (a cooperative launch whose iteration count the host controls).
- * Instances may differ in mesh, velocity grid, gas, walls and mode; they
+ * and replayed per step.  The synthetic solves of instances in mode M1 run
+ * either inside the graph, all in one launch with one workgroup per
+ * instance, or after the graph, one cooperative launch per instance (the
+ * only way for instances with a closure network or more than 256 swept rows).
+ * Inside is the default when at least 32 instances qualify; gsis_batch_tune
+ * times the other placement too, gsis_batch_set_ns_graph chooses it.  Both
+ * give the same results.  Instances may differ in mesh, velocity grid, gas, walls and mode; they
  * must share precision, flux scheme and device, and fit the
  * column-wavefront kernel.  A batch step keeps no kinetic RHS (gsis_residual
  * reports the last one a gsis_step kept).  The handles must not be stepped
@@ -327,9 +330,16 @@ int gsis_batch_create(gsis_handle *const *hs, int32_t n, gsis_batch **out);
 int gsis_batch_step(gsis_batch *b, int32_t nsteps);
 /* Autotuner: runs steps_per_trial real batch steps with each strip width
  * (8 or 16 columns per workgroup) and job order (as given, or the largest
- * kinetic work first), keeps the fastest (every choice computes the same
- * step) and reports it with its time per step. */
+ * kinetic work first), then with the best of those and the other placement
+ * of the synthetic solves (when any qualifies), keeps the fastest (every
+ * choice computes the same step) and reports it with its time per step. */
 int gsis_batch_tune(gsis_batch *b, int32_t steps_per_trial, int32_t *nw, int32_t *order, double *ms_per_step);
+/* Placement of the M1 instances' synthetic solves: -1 automatic (inside the
+ * graph when at least 32 instances qualify), 0 after the graph, 1 inside the
+ * graph for every instance that qualifies; the step is captured again. */
+int gsis_batch_set_ns_graph(gsis_batch *b, int32_t mode);
+/* The number of instances whose synthetic solve runs inside the graph. */
+int gsis_batch_ns_graph(const gsis_batch *b, int32_t *count);
 int gsis_batch_destroy(gsis_batch *b);
 
 #ifdef __cplusplus

@@ -75,10 +75,12 @@ def test_batch_equals_single_instances(prec):
         s.close()
 
 
-def test_batch_runs_m1_instances_like_single_steps():
+@pytest.mark.parametrize("ns_graph", [-1, 0, 1])
+def test_batch_runs_m1_instances_like_single_steps(ns_graph):
     """M1 instances in a batch: the captured part of the step, then each
-    instance's synthetic solve; the same fields and inner iteration counts as
-    stepping each instance on its own."""
+    instance's synthetic solve (inside the graph, one workgroup each, or after
+    it, one cooperative launch each); the same fields and inner iteration
+    counts as stepping each instance on its own."""
     g = load_pkg()
     from gsis_amd import batch
     cases = batch.instances(3, seed=7, side=(12, 24), nv=(8, 16), precision_scheme=g.capi.SCHEME_UP1)
@@ -89,6 +91,9 @@ def test_batch_runs_m1_instances_like_single_steps():
             s.initialize()
             lst.append(s)
     b = batch.Batch(grouped)
+    b.set_ns_graph(ns_graph)
+    # automatic: three instances are fewer than the 32 that put the solves in the graph
+    assert b.ns_graph() == (3 if ns_graph == 1 else 0)
     for k in range(3):
         b.step(1)
         for s, t in zip(single, grouped):
