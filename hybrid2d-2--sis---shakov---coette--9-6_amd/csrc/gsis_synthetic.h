@@ -39,7 +39,6 @@ struct SyntheticState {
   double *rec = nullptr;     // [np][nw][kNRec/2][64][2] sweep inputs, see NsRec
   double *dstar = nullptr;   // [np][nw][2][64][2] dU after the forward sweep (components in pairs)
   double *part = nullptr;    // [nflux][8]: per flux workgroup {iteration done (int), T<0.1 (int), max |Res(4)|}
-  double *dump = nullptr;    // [256] sink of the sweep's stores for lanes without a cell
   int *ctl = nullptr;        // [1] go (next iteration or -1), [3] a wait timed out
   int *iters = nullptr;      // [0]: inner iterations of the last step, [1]: T<0.1 flag,
                              // [2]: the launch stopped at kstop with the loop unfinished,

@@ -93,6 +93,16 @@ __device__ __forceinline__ double2 ld_pair(__amdgpu_buffer_rsrc_t r, unsigned vo
   return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, kCoherent));
 }
 
+// Every record slot's density field to 1 (the rest stays zero): the slots of
+// cells that are not swept keep this finite state for good.
+__global__ void k_rec_init(SyntheticState st) {
+  const size_t nslot = (size_t)st.np * st.nw * 64;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslot) return;
+  const size_t blk = t / 64, l = t % 64;
+  st.rec[blk * kBlk + (kRecU / 2) * 128 + 2 * l] = 1.0;
+}
+
 // S0 <- S - S0, S00 <- S on interior cells; the kinetic increment added to the
 // synthetic state of the previous step on cells 3..n-3 (src/Hybrid_Sub.f90:45-94).
 // Also resets the inner loop's control words.
@@ -358,8 +368,14 @@ __device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, 
     st_pair(rr, o + (kRecU / 2) * 1024, sh.U[0][lc], sh.U[1][lc]);
     st_pair(rr, o + (kRecU / 2 + 1) * 1024, sh.U[2][lc], sh.U[3][lc]);
     st_pair(rr, o + (kRecRa / 2) * 1024, 1.0 / alfa, 0.5 * lci + lvi);                // ra, cbi
-    st_pair(rr, o + (kRecCbj / 2) * 1024, 0.5 * lcj + lvj, 0.5 * gsi[g0]);  // cbj, hsi
-    st_pair(rr, o + (kRecHsj / 2) * 1024, 0.5 * gsj[g0], gni1[g0]);         // hsj, ni1
+    // si/2 and sj/2 weigh the forward sweep's terms across the cell's lower
+    // faces; on the first swept column (row) the cell across is not swept and
+    // the term is left out (src/sub_LU_SGS.f90:44-56): a zero weight leaves it
+    // out without a test in the sweep (the backward sweep reads these of cells
+    // i+1, j+1 >= 4 only)
+    const double hsi = i == 3 ? 0.0 : 0.5 * gsi[g0], hsj = j == 3 ? 0.0 : 0.5 * gsj[g0];
+    st_pair(rr, o + (kRecCbj / 2) * 1024, 0.5 * lcj + lvj, hsi);  // cbj, hsi
+    st_pair(rr, o + (kRecHsj / 2) * 1024, hsj, gni1[g0]);         // hsj, ni1
     st_pair(rr, o + (kRecNi / 2 + 1) * 1024, gni2[g0], gnj1[g0]);           // ni2, nj1
     st_pair(rr, o + (kRecNj / 2 + 1) * 1024, gnj2[g0], 0.0);                // nj2
   }
@@ -379,6 +395,20 @@ __device__ __forceinline__ double lane_shift(double x, bool up) {
   } else {
     lo = __builtin_amdgcn_mov_dpp(lo, 0x130, 0xf, 0xf, true);
     hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
+  }
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// the same, with 1.0 in the lane that has no such neighbour (a density)
+__device__ __forceinline__ double lane_shift_rho(double x, bool up) {
+  const unsigned long long b = __double_as_longlong(x), o = __double_as_longlong(1.0);
+  int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+  const int olo = (int)(unsigned)o, ohi = (int)(unsigned)(o >> 32);
+  if (up) {
+    lo = __builtin_amdgcn_update_dpp(olo, lo, 0x138, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(ohi, hi, 0x138, 0xf, 0xf, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(olo, lo, 0x130, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(ohi, hi, 0x130, 0xf, 0xf, false);
   }
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
@@ -570,7 +600,11 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     }
     // off-diagonal term from the j-neighbour
     dfn(nu, ndu, x.nj[0], x.nj[1], g, dfj);
-    const bool usei = FWD ? i != 3 : i != nx - 3, usej = FWD ? j != 3 : j != ny - 3;
+    // The terms across faces to cells that are not swept (i = 3, j = 3 forward;
+    // i = nx-3, j = ny-3 backward) vanish without a test: the forward weights
+    // Hsi, Hsj of those cells are zero, and a slot of a cell that is not swept
+    // holds U = (1,0,0,0) and zero weights (written once by k_rec_init, never by
+    // the flux workgroups), so its dU stays zero and its F.n(U) finite.
     double du[4];
     for (int m = 0; m < 4; m++) {
       // forward:  dui = 0.5 (DF si(i,j) + Lci(i-1,j) dU) + Lvi(i-1,j) dU      (:44-49)
@@ -578,8 +612,8 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       const double ti = FWD ? fma(x.hsi, dfi[m], ccbi * cdu[m]) : fma(-chsi, dfi[m], ccbi * cdu[m]);
       const double tj = FWD ? fma(x.hsj, dfj[m], ncb * ndu[m]) : fma(-nhs, dfj[m], ncb * ndu[m]);
       double a = FWD ? x.res[m] : 0.0;
-      a = usei ? a + ti : a;
-      a = usej ? a + tj : a;
+      a = a + ti;
+      a = a + tj;
       du[m] = FWD ? a * x.ra : x.res[m] + a * x.ra;  // backward: x.res holds dU*
     }
     // plain stores: dU* is read back by this lane, U is written back to the
@@ -600,7 +634,9 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     for (int m = 0; m < 4; m++) {
       cu[m] = x.u[m];
       cdu[m] = on ? du[m] : 0.0;
-      nu[m] = lane_shift(cu[m], FWD);
+      // (the edge lane of a wavefront without a neighbour gets the state
+      // (1,0,0,0), dU = 0: finite, and its term has a zero weight)
+      nu[m] = m == 0 ? lane_shift_rho(cu[m], FWD) : lane_shift(cu[m], FWD);
       ndu[m] = lane_shift(cdu[m], FWD);
     }
     ccbi = x.cbi;
@@ -921,7 +957,7 @@ int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st) {
     g_syn_err = "synthetic solve: the mesh is too large for the sweep's buffers (2 GiB)";
     return GSIS_EUNSUPPORTED;
   }
-  const size_t n_dbl = 4 * N + (size_t)kNRec * nq + 4 * nq + 8 * (size_t)std::max(st.nflux, 1) + 256;
+  const size_t n_dbl = 4 * N + (size_t)kNRec * nq + 4 * nq + 8 * (size_t)std::max(st.nflux, 1);
   const size_t bytes = n_dbl * 8 + 64;
   void *mem = nullptr;
   if (hipMalloc(&mem, bytes) != hipSuccess || hipMemset(mem, 0, bytes) != hipSuccess) {
@@ -933,10 +969,18 @@ int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st) {
   st.rec = b; b += kNRec * nq;
   st.dstar = b; b += 4 * nq;
   st.part = b; b += 8 * (size_t)std::max(st.nflux, 1);
-  st.dump = b; b += 256;
   st.ctl = reinterpret_cast<int *>(b);
   st.iters = st.ctl + 8;
   st.mem = mem;
+  if (nq > 0) {
+    hipLaunchKernelGGL(k_rec_init, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, 0, st);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+      (void)hipFree(mem);
+      st.mem = nullptr;
+      g_syn_err = "synthetic solve: record initialisation failed";
+      return GSIS_EHIP;
+    }
+  }
   st.tol = p.ns_tol;
   st.maxit = p.ns_maxit > 0 ? p.ns_maxit : 100000;
   st.local_dt = p.local_dt;
