@@ -480,8 +480,8 @@ __device__ __forceinline__ void wait_lds(int *p, int want, int *abort, int *err)
 // Lane j owns row j (j = 3 + thread index); step p visits plane i + j = p.
 // Each step is straight-line code: every lane issues the same loads and
 // stores (a lane without a cell stores out of the buffer's range, which
-// discards the store), so the waits for a record loaded kDepth steps earlier
-// do not wait for the later ones.  The loop runs a multiple of kDepth steps;
+// discards the store), so the waits for a record loaded kDepth - 1 steps
+// earlier do not wait for the later ones.  The loop runs a multiple of kDepth steps;
 // the steps after the last plane have no cell in any row.
 // kDepth records in registers: each step's record is loaded at the start of
 // the step kDepth - 1 steps before it is used (the loads come from the
