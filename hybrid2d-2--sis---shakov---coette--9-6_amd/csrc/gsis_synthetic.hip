@@ -50,6 +50,7 @@ constexpr int kTC = kTW * kTW;
 constexpr int kNFace = (kTile + 1) * kTile;  // i-faces (or j-faces) of a tile
 constexpr int kMaxSweepWaves = 16;           // rows <= 1024
 constexpr int kRingMax = 10;                 // sweep hand-off slots per wavefront (at most)
+constexpr int kSlot = 14;                    // doubles per hand-off slot: dU, dFx, dFy, Cbj, Hsj
 constexpr long long kWaitTicks = 200000000;  // bounded waits: 2 s of the 100 MHz wall clock
 
 enum { kCtlGo = 1, kCtlErr = 3 };
@@ -182,10 +183,14 @@ __device__ void van_leer(const double ql[4], const double qr[4], double g, doubl
   for (int m = 0; m < 4; m++) f[m] = fp[m] + fm[m];
 }
 
-// comput_DFn (src/sub_LU_SGS.f90:122-143): F.n(U + dU) - F.n(U) on the unit
-// normal (n1, n2), each half formed as the reference forms it.  1/rho of each
-// state is shared by its normal velocity and pressure (hardware reciprocal
-// and two Newton steps, about 1 ulp).
+// comput_DFn (src/sub_LU_SGS.f90:122-143): DF = F.n(U + dU) - F.n(U) on the
+// unit normal (n1, n2).  F.n is linear in n: F.n = Fx n1 + Fy n2 with the
+// Euler fluxes Fx, Fy of the state, so a cell's DF on any normal is
+// (Fx(U+dU) - Fx(U)) n1 + (Fy(U+dU) - Fy(U)) n2.  The sweep forms the two
+// differences once per cell, when its dU is known, and both cells that take
+// it as a neighbour (its i+1 and its j+1 forward, i-1 and j-1 backward)
+// project them on their own normals.  1/rho is shared by the velocities and
+// the pressure (hardware reciprocal and two Newton steps, about 1 ulp).
 __device__ __forceinline__ double rcp_nr(double a) {
   double r = __builtin_amdgcn_rcp(a);
   double e = fma(-a, r, 1.0);
@@ -193,27 +198,27 @@ __device__ __forceinline__ double rcp_nr(double a) {
   e = fma(-a, r, 1.0);
   return fma(r, e, r);
 }
-__device__ __forceinline__ void dfn(const double u[4], const double du[4], double n1, double n2, double g,
-                                    double df[4]) {
-  double u2[4];
-  for (int m = 0; m < 4; m++) u2[m] = u[m] + du[m];
-  const double r1 = rcp_nr(u[0]), r2 = rcp_nr(u2[0]);
-  const double un1 = (u[1] * n1 + u[2] * n2) * r1;
-  const double p1 = (g - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) * r1);
-  const double un2 = (u2[1] * n1 + u2[2] * n2) * r2;
-  const double p2 = (g - 1.0) * (u2[3] - 0.5 * (u2[1] * u2[1] + u2[2] * u2[2]) * r2);
-  df[0] = u2[0] * un2 - u[0] * un1;
-  df[1] = (u2[1] * un2 + p2 * n1) - (u[1] * un1 + p1 * n1);
-  df[2] = (u2[2] * un2 + p2 * n2) - (u[2] * un1 + p1 * n2);
-  df[3] = (u2[3] + p2) * un2 - (u[3] + p1) * un1;
+__device__ __forceinline__ void fxy(const double u[4], double g, double fx[4], double fy[4]) {
+  const double r = rcp_nr(u[0]);
+  const double vx = u[1] * r, vy = u[2] * r;
+  const double p = (g - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) * r);
+  const double e = u[3] + p;
+  fx[0] = u[1];
+  fx[1] = u[1] * vx + p;
+  fx[2] = u[2] * vx;
+  fx[3] = e * vx;
+  fy[0] = u[2];
+  fy[1] = u[1] * vy;
+  fy[2] = u[2] * vy + p;
+  fy[3] = e * vy;
 }
 
 struct NsShared {
   // sweep: lane 63 (forward) or lane 0 (backward) of wavefront w hands its
   // row's values of step s to the neighbouring wavefront in slot s % kRing,
-  // at xw[(w * kRingMax + slot) * 10] (first in the struct: slot addresses
-  // then fit the LDS instructions' offset fields)
-  double xw[kMaxSweepWaves * kRingMax * 10];
+  // at xw[(w * kRingMax + slot) * kSlot] (first in the struct: slot
+  // addresses then fit the LDS instructions' offset fields)
+  double xw[kMaxSweepWaves * kRingMax * kSlot];
   double U[4][kTC];    // tile + halo
   double pr[5][kTC];   // d, u, v, T, p (comput_duvtpckw)
   double mu[kTC];      // amu_of
@@ -398,21 +403,6 @@ __device__ __forceinline__ double lane_shift(double x, bool up) {
   }
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
-// the same, with 1.0 in the lane that has no such neighbour (a density)
-__device__ __forceinline__ double lane_shift_rho(double x, bool up) {
-  const unsigned long long b = __double_as_longlong(x), o = __double_as_longlong(1.0);
-  int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
-  const int olo = (int)(unsigned)o, ohi = (int)(unsigned)(o >> 32);
-  if (up) {
-    lo = __builtin_amdgcn_update_dpp(olo, lo, 0x138, 0xf, 0xf, false);
-    hi = __builtin_amdgcn_update_dpp(ohi, hi, 0x138, 0xf, 0xf, false);
-  } else {
-    lo = __builtin_amdgcn_update_dpp(olo, lo, 0x130, 0xf, 0xf, false);
-    hi = __builtin_amdgcn_update_dpp(ohi, hi, 0x130, 0xf, 0xf, false);
-  }
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
 // One sweep step's record of a lane's own cell (all of it in the forward
 // sweep; the backward sweep leaves res and reads dU* instead).
 struct Rec {
@@ -489,10 +479,11 @@ __device__ __forceinline__ void wait_lds(int *p, int want, int *abort, int *err)
 // step's arithmetic), which takes (kDepth - 1) * 34 more VGPRs than one
 // record.
 //
-// What a lane carries from one step to the next: its own cell's U, dU, Cbi
-// and Hsi (the cell is the i-neighbour of the lane's next one), and the
-// j-neighbour row's U, dU, Cbj and Hsj, shifted in from the next lane at the
-// end of the step (in the edge lane: from the neighbouring wavefront, below).
+// What a lane carries from one step to the next: its own cell's dU, flux
+// differences dFx, dFy, Cbi and Hsi (the cell is the i-neighbour of the
+// lane's next one), and the j-neighbour row's dU, dFx, dFy, Cbj and Hsj,
+// shifted in from the next lane at the end of the step (in the edge lane:
+// from the neighbouring wavefront, below).
 //
 // Hand-offs between wavefronts (LDS ring of kRing slots per wavefront): the
 // edge lane (63 forward, 0 backward) of wavefront w writes its values of step
@@ -551,19 +542,20 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   };
   // byte offset of cell (p - j, j) in a component of U, less 8 p
   const unsigned ucell = (unsigned)(((j + 1) * d.NI + 1 - j) * (int)sizeof(double));
-  // own previous cell (the i-neighbour) and the j-neighbour row, both of the previous step
-  double cu[4], cdu[4], ccbi = 0.0, chsi = 0.0;
-  double nu[4], ndu[4], ncb = 0.0, nhs = 0.0;
-  for (int m = 0; m < 4; m++) { cu[m] = nu[m] = 1.0; cdu[m] = ndu[m] = 0.0; }
+  // own previous cell (the i-neighbour) and the j-neighbour row, both of the
+  // previous step: dU, the flux differences dFx, dFy, and the weights
+  double cdu[4], cfx[4], cfy[4], ccbi = 0.0, chsi = 0.0;
+  double ndu[4], nfx[4], nfy[4], ncb = 0.0, nhs = 0.0;
+  for (int m = 0; m < 4; m++) cdu[m] = cfx[m] = cfy[m] = ndu[m] = nfx[m] = nfy[m] = 0.0;
   // the neighbour's slot at xb into the edge lane of the j-neighbour row
   auto read_slot = [&](const double *xb) {
     if (edge_in) {
-      for (int m = 0; m < 4; m++) { nu[m] = xb[m]; ndu[m] = xb[4 + m]; }
-      ncb = xb[8];
-      nhs = xb[9];
+      for (int m = 0; m < 4; m++) { ndu[m] = xb[m]; nfx[m] = xb[4 + m]; nfy[m] = xb[8 + m]; }
+      ncb = xb[12];
+      nhs = xb[13];
     }
   };
-  auto slot_of = [&](int v, int t) { return (v * kRingMax + t % kRing) * 10; };
+  auto slot_of = [&](int v, int t) { return (v * kRingMax + t % kRing) * kSlot; };
   int fprog = 0, fcons = 0;  // prog[from] and cons[to] as read one step earlier
   Rec r[kDepth];
 #pragma unroll
@@ -572,7 +564,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   // by one plane per step and held at the last plane; block of step s
   const int dbl = FWD ? nwr : -nwr, blim = FWD ? (nstep - 1) * nwr + w : w;
   int bl = bidx(kDepth - 1), bs = w;
-  // slots of the group of steps s0 .. s0+kDepth-1: xin + 10 q (from's), xout + 10 q (this wavefront's)
+  // slots of the group of steps s0 .. s0+kDepth-1: xin + kSlot q (from's), xout + kSlot q (this wavefront's)
   const double *xin = sh.xw;
   double *xout = sh.xw;
   auto step = [&](int s, int q, Rec &x) {
@@ -584,7 +576,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     const bool on = act(s);
     // off-diagonal term from the i-neighbour (this lane's previous cell)
     double dfi[4], dfj[4];
-    dfn(cu, cdu, x.ni[0], x.ni[1], g, dfi);
+    for (int m = 0; m < 4; m++) dfi[m] = fma(cfy[m], x.ni[1], cfx[m] * x.ni[0]);
     if (s > 0) {
       lgkm_wait();  // the previous step's slot write and reads are complete
       if (has_to && lane == 0) lds_put(pw, s);
@@ -599,12 +591,12 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       }
     }
     // off-diagonal term from the j-neighbour
-    dfn(nu, ndu, x.nj[0], x.nj[1], g, dfj);
+    for (int m = 0; m < 4; m++) dfj[m] = fma(nfy[m], x.nj[1], nfx[m] * x.nj[0]);
     // The terms across faces to cells that are not swept (i = 3, j = 3 forward;
     // i = nx-3, j = ny-3 backward) vanish without a test: the forward weights
     // Hsi, Hsj of those cells are zero, and a slot of a cell that is not swept
     // holds U = (1,0,0,0) and zero weights (written once by k_rec_init, never by
-    // the flux workgroups), so its dU stays zero and its F.n(U) finite.
+    // the flux workgroups), so its dU and flux differences stay zero.
     double du[4];
     for (int m = 0; m < 4; m++) {
       // forward:  dui = 0.5 (DF si(i,j) + Lci(i-1,j) dU) + Lvi(i-1,j) dU      (:44-49)
@@ -630,14 +622,24 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x.u[m] + du[m]), ru, v,
                                               m * N * (int)sizeof(double), 0);
     }
-    // carry: this cell, and the j-neighbour row of the next step
-    for (int m = 0; m < 4; m++) {
-      cu[m] = x.u[m];
-      cdu[m] = on ? du[m] : 0.0;
-      // (the edge lane of a wavefront without a neighbour gets the state
-      // (1,0,0,0), dU = 0: finite, and its term has a zero weight)
-      nu[m] = m == 0 ? lane_shift_rho(cu[m], FWD) : lane_shift(cu[m], FWD);
-      ndu[m] = lane_shift(cdu[m], FWD);
+    // carry: this cell's dU and flux differences, and the j-neighbour row's
+    // of the next step (zero in the edge lane of a wavefront without a
+    // neighbour, where the term has a zero weight)
+    {
+      double u2[4], fx1[4], fy1[4], fx2[4], fy2[4];
+      for (int m = 0; m < 4; m++) {
+        cdu[m] = on ? du[m] : 0.0;
+        u2[m] = x.u[m] + cdu[m];
+      }
+      fxy(x.u, g, fx1, fy1);
+      fxy(u2, g, fx2, fy2);
+      for (int m = 0; m < 4; m++) {
+        cfx[m] = fx2[m] - fx1[m];
+        cfy[m] = fy2[m] - fy1[m];
+        ndu[m] = lane_shift(cdu[m], FWD);
+        nfx[m] = lane_shift(cfx[m], FWD);
+        nfy[m] = lane_shift(cfy[m], FWD);
+      }
     }
     ccbi = x.cbi;
     chsi = x.hsi;
@@ -649,10 +651,10 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
         mem_order();
       }
       if (edge_out) {
-        double *xb = xout + 10 * q;
-        for (int m = 0; m < 4; m++) { xb[m] = cu[m]; xb[4 + m] = cdu[m]; }
-        xb[8] = x.cbj;
-        xb[9] = x.hsj;
+        double *xb = xout + kSlot * q;
+        for (int m = 0; m < 4; m++) { xb[m] = cdu[m]; xb[4 + m] = cfx[m]; xb[8 + m] = cfy[m]; }
+        xb[12] = x.cbj;
+        xb[13] = x.hsj;
       }
       mem_order();
       fcons = lds_get(cto);
@@ -661,7 +663,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       mem_order();
       fprog = lds_get(pfrom);
       mem_order();
-      read_slot(xin + 10 * q);
+      read_slot(xin + kSlot * q);
     }
     bs += nwr;
   };
