@@ -212,6 +212,22 @@ __device__ __forceinline__ void fxy(const double u[4], double g, double fx[4], d
   fy[2] = u[2] * vy + p;
   fy[3] = e * vy;
 }
+// DF of state u and increment du on (n1, n2) in the reference's form (the
+// sweep of the launch for more than 256 swept rows carries states)
+__device__ __forceinline__ void dfn(const double u[4], const double du[4], double n1, double n2, double g,
+                                    double df[4]) {
+  double u2[4];
+  for (int m = 0; m < 4; m++) u2[m] = u[m] + du[m];
+  const double r1 = rcp_nr(u[0]), r2 = rcp_nr(u2[0]);
+  const double un1 = (u[1] * n1 + u[2] * n2) * r1;
+  const double p1 = (g - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) * r1);
+  const double un2 = (u2[1] * n1 + u2[2] * n2) * r2;
+  const double p2 = (g - 1.0) * (u2[3] - 0.5 * (u2[1] * u2[1] + u2[2] * u2[2]) * r2);
+  df[0] = u2[0] * un2 - u[0] * un1;
+  df[1] = (u2[1] * un2 + p2 * n1) - (u[1] * un1 + p1 * n1);
+  df[2] = (u2[2] * un2 + p2 * n2) - (u[2] * un1 + p1 * n2);
+  df[3] = (u2[3] + p2) * un2 - (u[3] + p1) * un1;
+}
 
 struct NsShared {
   // sweep: lane 63 (forward) or lane 0 (backward) of wavefront w hands its
@@ -403,6 +419,21 @@ __device__ __forceinline__ double lane_shift(double x, bool up) {
   }
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
+// the same, with 1.0 in the lane that has no such neighbour (a density)
+__device__ __forceinline__ double lane_shift_rho(double x, bool up) {
+  const unsigned long long b = __double_as_longlong(x), o = __double_as_longlong(1.0);
+  int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+  const int olo = (int)(unsigned)o, ohi = (int)(unsigned)(o >> 32);
+  if (up) {
+    lo = __builtin_amdgcn_update_dpp(olo, lo, 0x138, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(ohi, hi, 0x138, 0xf, 0xf, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(olo, lo, 0x130, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(ohi, hi, 0x130, 0xf, 0xf, false);
+  }
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // One sweep step's record of a lane's own cell (all of it in the forward
 // sweep; the backward sweep leaves res and reads dU* instead).
 struct Rec {
@@ -483,7 +514,12 @@ __device__ __forceinline__ void wait_lds(int *p, int want, int *abort, int *err)
 // differences dFx, dFy, Cbi and Hsi (the cell is the i-neighbour of the
 // lane's next one), and the j-neighbour row's dU, dFx, dFy, Cbj and Hsj,
 // shifted in from the next lane at the end of the step (in the edge lane:
-// from the neighbouring wavefront, below).
+// from the neighbouring wavefront, below).  The launch for more than 256
+// swept rows (kDiff false) carries the states U instead and forms DF in the
+// reference's form: with flux differences that launch, held to 128 VGPRs and
+// spilling, gave results off by about 1e-6 on meshes of 294-302 swept rows,
+// varying from run to run, for a reason not found; with states it matches
+// the oracle on every tall mesh tried.
 //
 // Hand-offs between wavefronts (LDS ring of kRing slots per wavefront): the
 // edge lane (63 forward, 0 backward) of wavefront w writes its values of step
@@ -501,7 +537,7 @@ __device__ __forceinline__ void wait_lds(int *p, int want, int *abort, int *err)
 //    wavefront so settles one to two steps behind its neighbour;
 //  * a slot is written only when the word cons read one step earlier says it
 //    has been taken (cons only grows), else after waiting for it.
-template <bool FWD, int kDepth>
+template <bool FWD, int kDepth, bool kDiff>
 __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh) {
   const int nx = d.nx, ny = d.ny, N = d.ncell, nwr = st.nw;
   const double g = d.gamma;
@@ -543,14 +579,22 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   // byte offset of cell (p - j, j) in a component of U, less 8 p
   const unsigned ucell = (unsigned)(((j + 1) * d.NI + 1 - j) * (int)sizeof(double));
   // own previous cell (the i-neighbour) and the j-neighbour row, both of the
-  // previous step: dU, the flux differences dFx, dFy, and the weights
+  // previous step: dU and the weights, and either the flux differences dFx,
+  // dFy (kDiff) or the state U (cfx, nfx hold U; cfy, nfy unused)
   double cdu[4], cfx[4], cfy[4], ccbi = 0.0, chsi = 0.0;
   double ndu[4], nfx[4], nfy[4], ncb = 0.0, nhs = 0.0;
-  for (int m = 0; m < 4; m++) cdu[m] = cfx[m] = cfy[m] = ndu[m] = nfx[m] = nfy[m] = 0.0;
+  for (int m = 0; m < 4; m++) {
+    cdu[m] = cfy[m] = ndu[m] = nfy[m] = 0.0;
+    cfx[m] = nfx[m] = kDiff ? 0.0 : 1.0;
+  }
   // the neighbour's slot at xb into the edge lane of the j-neighbour row
   auto read_slot = [&](const double *xb) {
     if (edge_in) {
-      for (int m = 0; m < 4; m++) { ndu[m] = xb[m]; nfx[m] = xb[4 + m]; nfy[m] = xb[8 + m]; }
+      for (int m = 0; m < 4; m++) {
+        ndu[m] = xb[m];
+        nfx[m] = xb[4 + m];
+        if (kDiff) nfy[m] = xb[8 + m];
+      }
       ncb = xb[12];
       nhs = xb[13];
     }
@@ -576,7 +620,11 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     const bool on = act(s);
     // off-diagonal term from the i-neighbour (this lane's previous cell)
     double dfi[4], dfj[4];
-    for (int m = 0; m < 4; m++) dfi[m] = fma(cfy[m], x.ni[1], cfx[m] * x.ni[0]);
+    if constexpr (kDiff) {
+      for (int m = 0; m < 4; m++) dfi[m] = fma(cfy[m], x.ni[1], cfx[m] * x.ni[0]);
+    } else {
+      dfn(cfx, cdu, x.ni[0], x.ni[1], g, dfi);
+    }
     if (s > 0) {
       lgkm_wait();  // the previous step's slot write and reads are complete
       if (has_to && lane == 0) lds_put(pw, s);
@@ -591,7 +639,11 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       }
     }
     // off-diagonal term from the j-neighbour
-    for (int m = 0; m < 4; m++) dfj[m] = fma(nfy[m], x.nj[1], nfx[m] * x.nj[0]);
+    if constexpr (kDiff) {
+      for (int m = 0; m < 4; m++) dfj[m] = fma(nfy[m], x.nj[1], nfx[m] * x.nj[0]);
+    } else {
+      dfn(nfx, ndu, x.nj[0], x.nj[1], g, dfj);
+    }
     // The terms across faces to cells that are not swept (i = 3, j = 3 forward;
     // i = nx-3, j = ny-3 backward) vanish without a test: the forward weights
     // Hsi, Hsj of those cells are zero, and a slot of a cell that is not swept
@@ -625,7 +677,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     // carry: this cell's dU and flux differences, and the j-neighbour row's
     // of the next step (zero in the edge lane of a wavefront without a
     // neighbour, where the term has a zero weight)
-    {
+    if constexpr (kDiff) {
       double u2[4], fx1[4], fy1[4], fx2[4], fy2[4];
       for (int m = 0; m < 4; m++) {
         cdu[m] = on ? du[m] : 0.0;
@@ -640,6 +692,13 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
         nfx[m] = lane_shift(cfx[m], FWD);
         nfy[m] = lane_shift(cfy[m], FWD);
       }
+    } else {  // the state (the edge lane of a wavefront without a neighbour gets (1,0,0,0), dU = 0)
+      for (int m = 0; m < 4; m++) {
+        cfx[m] = x.u[m];
+        cdu[m] = on ? du[m] : 0.0;
+        nfx[m] = m == 0 ? lane_shift_rho(cfx[m], FWD) : lane_shift(cfx[m], FWD);
+        ndu[m] = lane_shift(cdu[m], FWD);
+      }
     }
     ccbi = x.cbi;
     chsi = x.hsi;
@@ -652,7 +711,11 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       }
       if (edge_out) {
         double *xb = xout + kSlot * q;
-        for (int m = 0; m < 4; m++) { xb[m] = cdu[m]; xb[4 + m] = cfx[m]; xb[8 + m] = cfy[m]; }
+        for (int m = 0; m < 4; m++) {
+          xb[m] = cdu[m];
+          xb[4 + m] = cfx[m];
+          if (kDiff) xb[8 + m] = cfy[m];
+        }
         xb[12] = x.cbj;
         xb[13] = x.hsj;
       }
@@ -677,18 +740,20 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
 
 // The wavefronts of a pass are coupled only through their hand-offs: a
 // wavefront may run up to 2 kDepth steps ahead of the one that takes its edge row.
-template <int kDepth>
+// kDiff: carry flux differences (the launches of 256 threads); the launch of
+// up to 1024 threads carries states (see sweep_pass)
+template <int kDepth, bool kDiff = true>
 __device__ void sweep(const Dev &d, const SyntheticState &st, NsShared &sh) {
   const int tid = threadIdx.x;
   if (tid < kMaxSweepWaves) { sh.prog[tid] = 0; sh.cons[tid] = 0; }
   if (tid == 0) sh.abort = 0;
   barrier_lds();
-  sweep_pass<true, kDepth>(d, st, sh);  // dU* = (Res + dui + duj)/alfa, planes 6 .. nx+ny-6 (:26-69)
+  sweep_pass<true, kDepth, kDiff>(d, st, sh);  // dU* = (Res + dui + duj)/alfa, planes 6 .. nx+ny-6 (:26-69)
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this lane's dU* stores are complete
   barrier_lds();
   if (tid < kMaxSweepWaves) { sh.prog[tid] = 0; sh.cons[tid] = 0; }
   barrier_lds();
-  sweep_pass<false, kDepth>(d, st, sh);  // dU = dU* + (dui + duj)/alfa, planes nx+ny-6 .. 6 (:73-111), then U + dU
+  sweep_pass<false, kDepth, kDiff>(d, st, sh);  // dU = dU* + (dui + duj)/alfa, planes nx+ny-6 .. 6 (:73-111), then U + dU
   // release: the U stores reach the device's coherence point before the
   // flux workgroups are told to read U (L2 write-back, then vmcnt(0))
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -815,7 +880,7 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
       res_last = res;
       __syncthreads();
       if (bad) break;
-      sweep<B <= 256 ? 4 : 3>(d, st, sh);  // deeper prefetch where registers allow
+      sweep<B <= 256 ? 4 : 3, B <= 256>(d, st, sh);  // deeper prefetch where registers allow
       __syncthreads();
       bool more = res > st.tol && k < st.maxit;
       if (more && k >= st.kstop) {  // the launch ends here; a later one continues at k + 1

@@ -163,6 +163,28 @@ def test_m1_skewed_periodic_box():
     cpu.close()
 
 
+def test_m1_tall_couette_parity():
+    """Mode M1 on a mesh with more than 256 swept rows (296): the inner loop
+    runs in the launch of up to 1024 threads, five sweep wavefronts handing
+    rows on, records three planes ahead.  Same fields and inner iteration
+    counts as the oracle.  This Couette's inner loop converges slowly (52 655
+    iterations in its first step, the same on both sides), so the loop is
+    capped at 400 iterations per step here."""
+    g = load_pkg()
+    case = g.cases.couette_case(ny_cells=300, nx_cells=12, nv=8, kn=0.1, uwall=0.1, dt=1.0)
+    print("case:", case.name, "M1", flush=True)
+    gpu, cpu = _pair(case, mode=1, ns_maxit=400)
+    for k in range(3):
+        gpu.step(1)
+        cpu.step(1)
+        ig, ic = gpu.counters()[2], cpu.counters()[2]
+        print("step", k + 1, "inner its gpu", ig, "oracle", ic, flush=True)
+        assert ig == ic > 1
+    _compare(gpu, cpu, "tall couette M1 step 3", frd=False)
+    gpu.close()
+    cpu.close()
+
+
 def test_m1_cavity_parity():
     g = load_pkg()
     case = g.cases.cavity_case(16, 24, kn=0.1, lid=0.15, dt=1.0)
