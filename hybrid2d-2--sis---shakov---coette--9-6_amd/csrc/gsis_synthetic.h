@@ -18,9 +18,9 @@ enum NsRec {
   kRecCbi = 9,   // Lci/2 + Lvi: the weight of dU in the i-direction off-diagonal terms
   kRecCbj = 10,  // Lcj/2 + Lvj
   kRecHsi = 11,  // si(i,j)/2
-  kRecHsj = 12,  // sj(i,j)/2
-  kRecNi = 13,   // ni1(i,j), ni2(i,j)
-  kRecNj = 15,   // nj1(i,j), nj2(i,j)
+  kRecHsj = 12,  // sj(i,j)/2 (field 13 unused: the normals fill the last two pairs)
+  kRecNi = 14,   // ni1(i,j), ni2(i,j)
+  kRecNj = 16,   // nj1(i,j), nj2(i,j)
   kNRec = 18     // 17 fields and one unused: nine pairs
 };
 
@@ -35,6 +35,7 @@ struct SyntheticState {
   int ntx = 0, nty = 0;      // flux tiles per direction
   int nflux = 0;             // flux workgroups of the inner-loop launch
   int block = 256;           // threads per workgroup of the inner-loop launch
+  bool axis = false;         // every swept cell's face normals are exactly (1,0) and (0,1)
   double *s00 = nullptr;     // [4][ncell]  S00_GKUA(1:4)
   double *rec = nullptr;     // [np][nw][kNRec/2][64][2] sweep inputs, see NsRec
   double *dstar = nullptr;   // [np][nw][2][64][2] dU after the forward sweep (components in pairs)

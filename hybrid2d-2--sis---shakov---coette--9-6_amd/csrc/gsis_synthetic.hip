@@ -62,7 +62,7 @@ enum { kCtlGo = 1, kCtlErr = 3 };
 // (synthetic_init), so byte offsets fit the 32-bit buffer offsets.
 constexpr int kBlk = kNRec * 64;  // doubles per block
 static_assert(kRecRes == 0 && kRecU == 4 && kRecRa == 8 && kRecCbi == 9 && kRecCbj == 10 && kRecHsi == 11 &&
-                  kRecHsj == 12 && kRecNi == 13 && kRecNj == 15 && kNRec == 18,
+                  kRecHsj == 12 && kRecNi == 14 && kRecNj == 16 && kNRec == 18,
               "the sweep's pair loads and the flux workgroups' pair stores assume this field order");
 __device__ __forceinline__ size_t rec_blk(int p, int w, int nw) { return ((size_t)(p - 6) * nw + w) * kBlk; }
 __device__ __forceinline__ size_t rec_cell(int i, int j, int nw) {  // the cell's first pair
@@ -396,9 +396,9 @@ __device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, 
     // i+1, j+1 >= 4 only)
     const double hsi = i == 3 ? 0.0 : 0.5 * gsi[g0], hsj = j == 3 ? 0.0 : 0.5 * gsj[g0];
     st_pair(rr, o + (kRecCbj / 2) * 1024, 0.5 * lcj + lvj, hsi);  // cbj, hsi
-    st_pair(rr, o + (kRecHsj / 2) * 1024, hsj, gni1[g0]);         // hsj, ni1
-    st_pair(rr, o + (kRecNi / 2 + 1) * 1024, gni2[g0], gnj1[g0]);           // ni2, nj1
-    st_pair(rr, o + (kRecNj / 2 + 1) * 1024, gnj2[g0], 0.0);                // nj2
+    st_pair(rr, o + (kRecHsj / 2) * 1024, hsj, 0.0);              // hsj
+    st_pair(rr, o + (kRecNi / 2) * 1024, gni1[g0], gni2[g0]);     // ni1, ni2
+    st_pair(rr, o + (kRecNj / 2) * 1024, gnj1[g0], gnj2[g0]);     // nj1, nj2
   }
   __syncthreads();  // the next tile reuses the LDS
 }
@@ -442,8 +442,10 @@ struct Rec {
 
 // Every lane loads its row's slot of the plane, whether or not the row has a
 // cell there (the slot exists for every row of a sweep wavefront): nine
-// 16-byte loads, one scalar block offset and immediate pair offsets.
-template <bool FWD>
+// 16-byte loads, one scalar block offset and immediate pair offsets.  On a
+// mesh whose normals are (1,0) and (0,1) (kAxis) the last two pairs, the
+// normals, are not loaded: seven loads.
+template <bool FWD, bool kAxis>
 __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t rr, int blk_bytes, __amdgpu_buffer_rsrc_t rd,
                                          int ds_bytes, int lane, Rec &x) {
   const unsigned v = (unsigned)lane * 16u;
@@ -456,15 +458,18 @@ __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t rr, int blk_byte
     r01 = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rd, v, ds_bytes, 0));
     r23 = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rd, v + 1024u, ds_bytes, 0));
   }
-  const double2 u01 = pr(kRecU / 2), u23 = pr(kRecU / 2 + 1), p4 = pr(4), p5 = pr(5), p6 = pr(6), p7 = pr(7),
-                p8 = pr(8);
+  const double2 u01 = pr(kRecU / 2), u23 = pr(kRecU / 2 + 1), p4 = pr(kRecRa / 2), p5 = pr(kRecCbj / 2),
+                p6 = pr(kRecHsj / 2);
   x.res[0] = r01.x; x.res[1] = r01.y; x.res[2] = r23.x; x.res[3] = r23.y;
   x.u[0] = u01.x; x.u[1] = u01.y; x.u[2] = u23.x; x.u[3] = u23.y;
   x.ra = p4.x; x.cbi = p4.y;
   x.cbj = p5.x; x.hsi = p5.y;
-  x.hsj = p6.x; x.ni[0] = p6.y;
-  x.ni[1] = p7.x; x.nj[0] = p7.y;
-  x.nj[1] = p8.x;
+  x.hsj = p6.x;
+  if (!kAxis) {
+    const double2 p7 = pr(kRecNi / 2), p8 = pr(kRecNj / 2);
+    x.ni[0] = p7.x; x.ni[1] = p7.y;
+    x.nj[0] = p8.x; x.nj[1] = p8.y;
+  }
 }
 
 // LDS words read and written by several wavefronts of the workgroup
@@ -537,8 +542,15 @@ __device__ __forceinline__ void wait_lds(int *p, int want, int *abort, int *err)
 //    wavefront so settles one to two steps behind its neighbour;
 //  * a slot is written only when the word cons read one step earlier says it
 //    has been taken (cons only grows), else after waiting for it.
-template <bool FWD, int kDepth, bool kDiff>
+//
+// kAxis (with kDiff): every swept cell's normals are ni = (1,0), nj = (0,1),
+// so DF across an i-face is dFx and across a j-face dFy, exactly the values
+// fma(dFy, ni2, dFx ni1) and fma(dFy, nj2, dFx nj1) take there (up to the
+// sign of a zero): the normals are not loaded, a lane hands on dFy of its row
+// and keeps dFx.
+template <bool FWD, int kDepth, bool kDiff, bool kAxis>
 __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh) {
+  static_assert(kDiff || !kAxis, "the axis-aligned form carries flux differences");
   const int nx = d.nx, ny = d.ny, N = d.ncell, nwr = st.nw;
   const double g = d.gamma;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -592,7 +604,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
     if (edge_in) {
       for (int m = 0; m < 4; m++) {
         ndu[m] = xb[m];
-        nfx[m] = xb[4 + m];
+        if (!kAxis) nfx[m] = xb[4 + m];
         if (kDiff) nfy[m] = xb[8 + m];
       }
       ncb = xb[12];
@@ -603,7 +615,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   int fprog = 0, fcons = 0;  // prog[from] and cons[to] as read one step earlier
   Rec r[kDepth];
 #pragma unroll
-  for (int q = 0; q < kDepth - 1; q++) load_rec<FWD>(rr, bidx(q) * kBlk * 8, rd, bidx(q) * 2048, lane, r[q]);
+  for (int q = 0; q < kDepth - 1; q++) load_rec<FWD, kAxis>(rr, bidx(q) * kBlk * 8, rd, bidx(q) * 2048, lane, r[q]);
   // block of the loads issued in step s (those of step s + kDepth - 1), moving
   // by one plane per step and held at the last plane; block of step s
   const int dbl = FWD ? nwr : -nwr, blim = FWD ? (nstep - 1) * nwr + w : w;
@@ -614,13 +626,15 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
   auto step = [&](int s, int q, Rec &x) {
     // the record of step s + kDepth - 1, into the slot step s - 1 has used:
     // issued first, so that the loads go out among this step's arithmetic
-    load_rec<FWD>(rr, bl * (kBlk * 8), rd, bl * 2048, lane, r[(q + kDepth - 1) % kDepth]);
+    load_rec<FWD, kAxis>(rr, bl * (kBlk * 8), rd, bl * 2048, lane, r[(q + kDepth - 1) % kDepth]);
     bl = FWD ? min(bl + dbl, blim) : max(bl + dbl, blim);
     const int p = pof(s), i = p - j;
     const bool on = act(s);
     // off-diagonal term from the i-neighbour (this lane's previous cell)
     double dfi[4], dfj[4];
-    if constexpr (kDiff) {
+    if constexpr (kAxis) {
+      for (int m = 0; m < 4; m++) dfi[m] = cfx[m];
+    } else if constexpr (kDiff) {
       for (int m = 0; m < 4; m++) dfi[m] = fma(cfy[m], x.ni[1], cfx[m] * x.ni[0]);
     } else {
       dfn(cfx, cdu, x.ni[0], x.ni[1], g, dfi);
@@ -639,7 +653,9 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
       }
     }
     // off-diagonal term from the j-neighbour
-    if constexpr (kDiff) {
+    if constexpr (kAxis) {
+      for (int m = 0; m < 4; m++) dfj[m] = nfy[m];
+    } else if constexpr (kDiff) {
       for (int m = 0; m < 4; m++) dfj[m] = fma(nfy[m], x.nj[1], nfx[m] * x.nj[0]);
     } else {
       dfn(nfx, ndu, x.nj[0], x.nj[1], g, dfj);
@@ -689,7 +705,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
         cfx[m] = fx2[m] - fx1[m];
         cfy[m] = fy2[m] - fy1[m];
         ndu[m] = lane_shift(cdu[m], FWD);
-        nfx[m] = lane_shift(cfx[m], FWD);
+        if (!kAxis) nfx[m] = lane_shift(cfx[m], FWD);
         nfy[m] = lane_shift(cfy[m], FWD);
       }
     } else {  // the state (the edge lane of a wavefront without a neighbour gets (1,0,0,0), dU = 0)
@@ -713,7 +729,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
         double *xb = xout + kSlot * q;
         for (int m = 0; m < 4; m++) {
           xb[m] = cdu[m];
-          xb[4 + m] = cfx[m];
+          if (!kAxis) xb[4 + m] = cfx[m];
           if (kDiff) xb[8 + m] = cfy[m];
         }
         xb[12] = x.cbj;
@@ -741,19 +757,20 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
 // The wavefronts of a pass are coupled only through their hand-offs: a
 // wavefront may run up to 2 kDepth steps ahead of the one that takes its edge row.
 // kDiff: carry flux differences (the launches of 256 threads); the launch of
-// up to 1024 threads carries states (see sweep_pass)
-template <int kDepth, bool kDiff = true>
+// up to 1024 threads carries states (see sweep_pass).  kAxis: the mesh's
+// normals are (1,0) and (0,1) on every swept cell.
+template <int kDepth, bool kDiff = true, bool kAxis = false>
 __device__ void sweep(const Dev &d, const SyntheticState &st, NsShared &sh) {
   const int tid = threadIdx.x;
   if (tid < kMaxSweepWaves) { sh.prog[tid] = 0; sh.cons[tid] = 0; }
   if (tid == 0) sh.abort = 0;
   barrier_lds();
-  sweep_pass<true, kDepth, kDiff>(d, st, sh);  // dU* = (Res + dui + duj)/alfa, planes 6 .. nx+ny-6 (:26-69)
+  sweep_pass<true, kDepth, kDiff, kAxis>(d, st, sh);  // dU* = (Res + dui + duj)/alfa, planes 6 .. nx+ny-6 (:26-69)
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this lane's dU* stores are complete
   barrier_lds();
   if (tid < kMaxSweepWaves) { sh.prog[tid] = 0; sh.cons[tid] = 0; }
   barrier_lds();
-  sweep_pass<false, kDepth, kDiff>(d, st, sh);  // dU = dU* + (dui + duj)/alfa, planes nx+ny-6 .. 6 (:73-111), then U + dU
+  sweep_pass<false, kDepth, kDiff, kAxis>(d, st, sh);  // dU = dU* + (dui + duj)/alfa, planes nx+ny-6 .. 6 (:73-111), then U + dU
   // release: the U stores reach the device's coherence point before the
   // flux workgroups are told to read U (L2 write-back, then vmcnt(0))
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -842,9 +859,11 @@ __device__ __attribute__((noinline)) void ns_flux_role(const Dev d_arg, const Sy
 }
 
 // The inner loop (src/Hybrid_Sub.f90:99-104): workgroup 0 sweeps, the others
-// compute the fluxes; see the file comment.
-template <int B>
+// compute the fluxes; see the file comment.  kAxis: the sweep of a mesh whose
+// normals are (1,0) and (0,1) on every swept cell (launches of 256 threads).
+template <int B, bool kAxis>
 __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
+  static_assert(B <= 256 || !kAxis, "the axis-aligned sweep carries flux differences");
   __shared__ NsShared sh;
   int *ctl = st.ctl;
   const int tid = threadIdx.x;
@@ -880,7 +899,7 @@ __global__ void __launch_bounds__(B) k_ns_loop(Dev d, SyntheticState st) {
       res_last = res;
       __syncthreads();
       if (bad) break;
-      sweep<B <= 256 ? 4 : 3, B <= 256>(d, st, sh);  // deeper prefetch where registers allow
+      sweep<B <= 256 ? 4 : 3, B <= 256, kAxis>(d, st, sh);  // deeper prefetch where registers allow
       __syncthreads();
       bool more = res > st.tol && k < st.maxit;
       if (more && k >= st.kstop) {  // the launch ends here; a later one continues at k + 1
@@ -987,6 +1006,26 @@ __global__ void k_ns_empty(SyntheticState st) {
   if (threadIdx.x == 0 && blockIdx.x == 0) st.iters[0] = st.ctl[kCtlGo] == 1 ? 1 : 0;
 }
 
+// the inner-loop kernel of an instance: 256 or 1024 threads, and for 256 the
+// sweep of an axis-aligned mesh where it is one
+static const void *ns_loop_kernel(const SyntheticState &st) {
+  if (st.block > 256) return reinterpret_cast<const void *>(k_ns_loop<1024, false>);
+  return st.axis ? reinterpret_cast<const void *>(k_ns_loop<256, true>)
+                 : reinterpret_cast<const void *>(k_ns_loop<256, false>);
+}
+
+// whether every swept cell (3..nx-3, 3..ny-3) has the face normals ni = (1,0)
+// and nj = (0,1) exactly (a zero of either sign), in the caller's Fortran
+// (nx, ny) arrays
+static bool axis_aligned(const gsis_params &p, int nx, int ny) {
+  for (int j = 3; j <= ny - 3; j++)
+    for (int i = 3; i <= nx - 3; i++) {
+      const size_t g = (size_t)(i - 1) + (size_t)nx * (j - 1);
+      if (!(p.ni1[g] == 1.0 && p.ni2[g] == 0.0 && p.nj1[g] == 0.0 && p.nj2[g] == 1.0)) return false;
+    }
+  return true;
+}
+
 int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st) {
   const size_t N = d.ncell;
   st.rs = d.ny - 5 > 0 && d.nx - 5 > 0 ? d.ny - 5 : 0;
@@ -999,13 +1038,13 @@ int synthetic_init(Dev &d, const gsis_params &p, SyntheticState &st) {
   st.nty = st.rs > 0 ? (st.rs + kTile - 1) / kTile : 0;
   const int ntiles = st.ntx * st.nty;
   st.block = std::max(256, 64 * ((st.rs + 63) / 64));
+  st.axis = st.block <= 256 && st.rs > 0 && axis_aligned(p, d.nx, d.ny);
   if (ntiles > 0) {
     int dev = 0, ncu = 0, per = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess)
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, st.block <= 256 ? k_ns_loop<256> : k_ns_loop<1024>,
-                                                       st.block, 0);
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ns_loop_kernel(st), st.block, 0);
     if (e != hipSuccess || per < 1) {
       g_syn_err = std::string("synthetic solve: occupancy query: ") + hipGetErrorString(e) + ", " +
                   std::to_string(per) + " workgroups of " + std::to_string(st.block) + " threads per CU";
@@ -1081,9 +1120,7 @@ int synthetic_run(const Dev &d, SyntheticState &st, int k0, int kstop, hipStream
     ss.k0 = k0;
     ss.kstop = kstop;
     void *args[] = {&dd, &ss};
-    const void *fn = st.block <= 256 ? reinterpret_cast<const void *>(k_ns_loop<256>)
-                                      : reinterpret_cast<const void *>(k_ns_loop<1024>);
-    const hipError_t e = hipLaunchCooperativeKernel(fn,
+    const hipError_t e = hipLaunchCooperativeKernel(ns_loop_kernel(st),
                                                     dim3(1 + st.nflux), dim3(st.block), args, 0, s);
     if (e != hipSuccess) {
       g_syn_err = std::string("synthetic solve: cooperative launch failed: ") + hipGetErrorString(e);
