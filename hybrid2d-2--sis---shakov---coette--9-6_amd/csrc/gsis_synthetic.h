@@ -18,10 +18,11 @@ enum NsRec {
   kRecCbi = 9,   // Lci/2 + Lvi: the weight of dU in the i-direction off-diagonal terms
   kRecCbj = 10,  // Lcj/2 + Lvj
   kRecHsi = 11,  // si(i,j)/2
-  kRecHsj = 12,  // sj(i,j)/2 (field 13 unused: the normals fill the last two pairs)
+  kRecHsj = 12,  // sj(i,j)/2
+  kRecRr = 13,   // 1/U(1), the reciprocal of the record's density (as the sweep forms it)
   kRecNi = 14,   // ni1(i,j), ni2(i,j)
   kRecNj = 16,   // nj1(i,j), nj2(i,j)
-  kNRec = 18     // 17 fields and one unused: nine pairs
+  kNRec = 18     // nine pairs
 };
 
 struct SyntheticState {

@@ -62,7 +62,7 @@ enum { kCtlGo = 1, kCtlErr = 3 };
 // (synthetic_init), so byte offsets fit the 32-bit buffer offsets.
 constexpr int kBlk = kNRec * 64;  // doubles per block
 static_assert(kRecRes == 0 && kRecU == 4 && kRecRa == 8 && kRecCbi == 9 && kRecCbj == 10 && kRecHsi == 11 &&
-                  kRecHsj == 12 && kRecNi == 14 && kRecNj == 16 && kNRec == 18,
+                  kRecHsj == 12 && kRecRr == 13 && kRecNi == 14 && kRecNj == 16 && kNRec == 18,
               "the sweep's pair loads and the flux workgroups' pair stores assume this field order");
 __device__ __forceinline__ size_t rec_blk(int p, int w, int nw) { return ((size_t)(p - 6) * nw + w) * kBlk; }
 __device__ __forceinline__ size_t rec_cell(int i, int j, int nw) {  // the cell's first pair
@@ -94,14 +94,15 @@ __device__ __forceinline__ double2 ld_pair(__amdgpu_buffer_rsrc_t r, unsigned vo
   return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, kCoherent));
 }
 
-// Every record slot's density field to 1 (the rest stays zero): the slots of
-// cells that are not swept keep this finite state for good.
+// Every record slot's density field and its reciprocal to 1 (the rest stays
+// zero): the slots of cells that are not swept keep this finite state for good.
 __global__ void k_rec_init(SyntheticState st) {
   const size_t nslot = (size_t)st.np * st.nw * 64;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nslot) return;
   const size_t blk = t / 64, l = t % 64;
   st.rec[blk * kBlk + (kRecU / 2) * 128 + 2 * l] = 1.0;
+  st.rec[blk * kBlk + (kRecRr / 2) * 128 + 2 * l + 1] = 1.0;
 }
 
 // S0 <- S - S0, S00 <- S on interior cells; the kinetic increment added to the
@@ -198,8 +199,8 @@ __device__ __forceinline__ double rcp_nr(double a) {
   e = fma(-a, r, 1.0);
   return fma(r, e, r);
 }
-__device__ __forceinline__ void fxy(const double u[4], double g, double fx[4], double fy[4]) {
-  const double r = rcp_nr(u[0]);
+// Fx(u), Fy(u) with r = 1/u[0] as rcp_nr forms it
+__device__ __forceinline__ void fxy_r(const double u[4], double r, double g, double fx[4], double fy[4]) {
   const double vx = u[1] * r, vy = u[2] * r;
   const double p = (g - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) * r);
   const double e = u[3] + p;
@@ -211,6 +212,9 @@ __device__ __forceinline__ void fxy(const double u[4], double g, double fx[4], d
   fy[1] = u[1] * vy;
   fy[2] = u[2] * vy + p;
   fy[3] = e * vy;
+}
+__device__ __forceinline__ void fxy(const double u[4], double g, double fx[4], double fy[4]) {
+  fxy_r(u, rcp_nr(u[0]), g, fx, fy);
 }
 // DF of state u and increment du on (n1, n2) in the reference's form (the
 // sweep of the launch for more than 256 swept rows carries states)
@@ -396,7 +400,7 @@ __device__ void flux_tile(const Dev &d, const SyntheticState &st, NsShared &sh, 
     // i+1, j+1 >= 4 only)
     const double hsi = i == 3 ? 0.0 : 0.5 * gsi[g0], hsj = j == 3 ? 0.0 : 0.5 * gsj[g0];
     st_pair(rr, o + (kRecCbj / 2) * 1024, 0.5 * lcj + lvj, hsi);  // cbj, hsi
-    st_pair(rr, o + (kRecHsj / 2) * 1024, hsj, 0.0);              // hsj
+    st_pair(rr, o + (kRecHsj / 2) * 1024, hsj, rcp_nr(sh.U[0][lc]));  // hsj, 1/rho
     st_pair(rr, o + (kRecNi / 2) * 1024, gni1[g0], gni2[g0]);     // ni1, ni2
     st_pair(rr, o + (kRecNj / 2) * 1024, gnj1[g0], gnj2[g0]);     // nj1, nj2
   }
@@ -437,7 +441,7 @@ __device__ __forceinline__ double lane_shift_rho(double x, bool up) {
 // One sweep step's record of a lane's own cell (all of it in the forward
 // sweep; the backward sweep leaves res and reads dU* instead).
 struct Rec {
-  double res[4], ra, ni[2], nj[2], hsi, hsj, u[4], cbi, cbj;
+  double res[4], ra, ni[2], nj[2], hsi, hsj, u[4], cbi, cbj, rr;
 };
 
 // Every lane loads its row's slot of the plane, whether or not the row has a
@@ -465,6 +469,7 @@ __device__ __forceinline__ void load_rec(__amdgpu_buffer_rsrc_t rr, int blk_byte
   x.ra = p4.x; x.cbi = p4.y;
   x.cbj = p5.x; x.hsi = p5.y;
   x.hsj = p6.x;
+  x.rr = p6.y;
   if (!kAxis) {
     const double2 p7 = pr(kRecNi / 2), p8 = pr(kRecNj / 2);
     x.ni[0] = p7.x; x.ni[1] = p7.y;
@@ -699,7 +704,7 @@ __device__ void sweep_pass(const Dev &d, const SyntheticState &st, NsShared &sh)
         cdu[m] = on ? du[m] : 0.0;
         u2[m] = x.u[m] + cdu[m];
       }
-      fxy(x.u, g, fx1, fy1);
+      fxy_r(x.u, x.rr, g, fx1, fy1);  // 1/rho of the record's state from the record
       fxy(u2, g, fx2, fy2);
       for (int m = 0; m < 4; m++) {
         cfx[m] = fx2[m] - fx1[m];
@@ -951,7 +956,10 @@ __global__ void __launch_bounds__(256) k_ns_batch(const NsJob *__restrict__ jobs
       bad = b;
       __syncthreads();
       if (bad) break;
-      sweep<4>(d, st, sh);
+      if (st.axis)
+        sweep<4, true, true>(d, st, sh);
+      else
+        sweep<4>(d, st, sh);
       __syncthreads();
       if (!(res > st.tol && k < st.maxit)) break;
     }
