@@ -3,8 +3,8 @@
  *
  * This file restates, in plain C and in the reference's own array layout
  * (Fortran column-major with the same ghost lower bounds), the FP64 algorithm
- * of the Fortran code Hybrid2D for one outer GSIS iteration.  It is the checker
- * the HIP path is compared against.  Only tests/, __graft_entry__.smoke() and
+ * of the Fortran code Hybrid2D for one outer GSIS iteration.  The HIP path is
+ * compared against it.  Only tests/, __graft_entry__.smoke() and
  * bench.py's CPU-baseline leg may load it; the product path never does.
  *
  * Every routine cites the reference lines it follows.  Single-precision

@@ -1,7 +1,7 @@
 /*
  * gsis_oracle.h -- CPU oracle of the GSIS kinetic loop.  TEST INFRASTRUCTURE ONLY:
  * loaded by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as the
- * checker, never by the product path.  Arrays use the reference's Fortran
+ * reference values, never by the product path.  Arrays use the reference's Fortran
  * layouts, documented in include/gsis_capi.h.
  */
 #ifndef GSIS_ORACLE_H

@@ -1,6 +1,6 @@
 """CPU FP64 restatement of the learned HoT closure's network.  TEST
-INFRASTRUCTURE ONLY: the checker of the GPU training and inference kernels;
-the product never imports it.
+INFRASTRUCTURE ONLY: what the GPU training and inference kernels are checked
+against; the product never imports it.
 
 The network (no counterpart in the reference) maps 8 standardised inputs
 through 4 hidden layers of 128 SiLU units to 5 standardised outputs; the

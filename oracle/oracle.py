@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (gsis_oracle.c).  TEST INFRASTRUCTURE ONLY.
 
 Loaded by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as
-the checker; the product path (the GsisSolver over libgsis.so) never imports
+the reference values; the product path (the GsisSolver over libgsis.so) never imports
 it.  OracleSolver offers the same methods as the product's GsisSolver so that
 both can be driven from the same inputs.
 """

@@ -1,6 +1,6 @@
-"""Pins of the CPU FP64 network (oracle/mlp.py), the checker of the GPU
-closure kernels: backpropagation against central finite differences, a
-network reduced by hand to a closed form, BF16 rounding against known
+"""Pins of the CPU FP64 network (oracle/mlp.py), which the GPU closure
+kernels are checked against: backpropagation against central finite
+differences, a network reduced by hand to a closed form, BF16 rounding against known
 values, Adam's first step.  Each check fails for a planted mistake in the
 oracle (a dropped bias, a transposed weight, a wrong SiLU derivative)."""
 import importlib.util
