@@ -23,7 +23,10 @@
 //    below (a cross-lane shift; between wavefronts a hand-off through an LDS
 //    ring, no barrier per plane).  The backward pass writes U + dU of each
 //    cell as soon as it is final; the sweeps read the pre-update U from the
-//    records, so the update is the reference's "after both sweeps".
+//    records, so the update is the reference's "after both sweeps".  On a
+//    mesh whose swept cells all have the normals (1,0) and (0,1) (checked
+//    on the host at set-up) the sweep skips the normals: DF across an
+//    i-face is dFx, across a j-face dFy.
 // Any order of the cells that respects the sweep dependences gives the same
 // dU, and the planes are visited in the reference's order, so the inner
 // iteration counts are the reference's.  The loop ends by the reference's
